@@ -1,0 +1,188 @@
+"""Benchmark: DCF evaluations per second on one MI355X.
+
+Workload (the reference's plain GPU configuration): DCF with bitWidthIn =
+bitWidthOut = 32, 2^24 elements, one key per element, groupSize 1, inputs and
+seeds uniform from a fixed seed, keys generated on the GPU.  One step = one
+party's FastFss_cuda_dcfEval over all 2^24 elements (one kernel launch).
+
+The JSON line also carries:
+  roofline      the eval kernel against the VALU integer peak (the bound; the
+                path has no dense contraction) with its HBM rate beside it
+  cpu_baseline  the C oracle (a port of the reference CPU path, AES-NI +
+                OpenMP) timed on this host on a bounded sample
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+REPO = Path(__file__).resolve().parent
+
+# MI355X: 256 CUs x 64 int32 lane-ops/clk x 2.4 GHz (published peak clock)
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
+HBM_PEAK_GBS = 8000.0
+# Algorithmic cost of one DCF eval (groupSize 1, taken child only) at n levels:
+# per level one AES-128 key schedule (10 rounds x 12 int ops) and two blocks
+# (10 rounds x 32 int ops: 16 byte extracts + 16 XORs on T-table lookups).
+OPS_PER_LEVEL = 10 * 12 + 2 * 10 * 32
+# HBM bytes per eval at n=32, u32: key 652 (sCW 512 + vCW 128 + tLCW, tRCW,
+# lastCW 12) + seed 16 + masked x 4 + share out 4.
+BYTES_PER_EVAL_N32 = 652 + 16 + 4 + 4
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--log2n", type=int, default=24)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-sample-log2", type=int, default=22)
+    return ap.parse_args()
+
+
+def cpu_baseline(log2n: int):
+    """Time the oracle (C port of the reference CPU path) on this host."""
+    import numpy as np
+
+    sys.path.insert(0, str(REPO / "tests"))
+    from fss_testlib import OracleBackend, dcf_keygen, ocall, oracle, rand_elems, rand_seeds
+
+    lib = oracle()
+    cores = min(16, os.cpu_count() or 1)
+    lib.oracle_setNumThreads(cores)
+    rng = np.random.default_rng(7)
+    N, n = 1 << log2n, 32
+    alpha = rand_elems(rng, N, n, 4)
+    x = rand_elems(rng, N, n, 4)
+    s0, s1 = rand_seeds(rng, N), rand_seeds(rng, N)
+    key = dcf_keygen(OracleBackend(), alpha, None, s0, s1, n, n, 1, 4)
+    out = np.zeros(N, dtype=np.uint32)
+    ocall("dcfEval", out, out.nbytes, x, x.nbytes, key, key.nbytes, s0, s0.nbytes, 0, n, n, 1, 4, N, None, 0)
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        rc = ocall("dcfEval", out, out.nbytes, x, x.nbytes, key, key.nbytes, s0, s0.nbytes, 0, n, n, 1, 4, N, None,
+                   0)
+        times.append(time.perf_counter() - t0)
+        assert rc == 0
+    t = sorted(times)[1]
+    return {
+        "value": N / t,
+        "unit": "evals/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"DCF n=32 groupSize 1, 2^{log2n} elements, party 0, median of 3 runs, "
+                  f"AES-NI={'yes' if lib.oracle_aes_uses_aesni() else 'no'}",
+    }
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": a.gpus}))
+        return 2
+
+    import torch
+
+    from fastfunctionsecretsharing_amd import _lib
+
+    lib = _lib.load()
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    stream = torch.cuda.current_stream(dev)
+    sp = ctypes.c_void_p(stream.cuda_stream)
+    spp = ctypes.addressof(sp)
+
+    N, n, m, g, es = 1 << a.log2n, 32, 32, 1, 4
+    gen = torch.Generator(device=dev).manual_seed(2024)
+    alpha = torch.randint(0, 2**32, (N,), dtype=torch.int64, device=dev, generator=gen)
+    x = torch.randint(0, 2**32, (N,), dtype=torch.int64, device=dev, generator=gen)
+    masked = ((x + alpha) & 0xFFFFFFFF).to(torch.int32)
+    a32 = alpha.to(torch.int32)
+    seeds = torch.randint(0, 256, (2, 16 * N), dtype=torch.uint8, device=dev, generator=gen)
+    ks = ctypes.c_size_t(0)
+    assert lib.FastFss_dcfGetKeyDataSize(ctypes.addressof(ks), n, m, g, es, N) == 0
+    key = torch.empty(ks.value, dtype=torch.uint8, device=dev)
+    rc = lib.FastFss_cuda_dcfKeyGen(key.data_ptr(), ks.value, a32.data_ptr(), 4 * N, None, 0, seeds[0].data_ptr(),
+                                    16 * N, seeds[1].data_ptr(), 16 * N, n, m, g, es, N, spp)
+    assert rc == 0, rc
+    out = torch.empty(N, dtype=torch.int32, device=dev)
+
+    def step(party=0):
+        r = lib.FastFss_cuda_dcfEval(out.data_ptr(), 4 * N, masked.data_ptr(), 4 * N, key.data_ptr(), ks.value,
+                                     seeds[party].data_ptr(), 16 * N, party, n, m, g, es, N, None, 0, spp)
+        assert r == 0, r
+
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    # correctness guard on the measured configuration: both parties reconstruct [x < alpha]
+    step(0)
+    y0 = out.clone()
+    step(1)
+    rec = (y0.to(torch.int64) + out.to(torch.int64)) & 0xFFFFFFFF
+    ok = bool(torch.equal(rec, (((x + alpha) & 0xFFFFFFFF) < alpha).to(torch.int64)))
+    torch.cuda.synchronize()
+
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(a.steps):
+        step()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = ev0.elapsed_time(ev1) / a.steps
+    evals_per_s = N * a.steps / wall
+
+    ops = n * OPS_PER_LEVEL * N
+    achieved_tops = ops / (kernel_ms * 1e-3) / 1e12
+    hbm_gbs = BYTES_PER_EVAL_N32 * N / (kernel_ms * 1e-3) / 1e9
+
+    res = {
+        "metric": "DCF evals/s (n=32, 2^24 elems), ragged-batch vs plain; p50/p99 latency",
+        "value": evals_per_s,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": wall * 1e3 / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (uniform alpha, x, seeds from a fixed torch seed; keys from FastFss_cuda_dcfKeyGen)",
+        "config": {"workload": "plain DCF eval, bitWidthIn=32, bitWidthOut=32, groupSize=1, 2^%d elements, "
+                               "one key per element, one party per step" % a.log2n,
+                   "elements": N, "bitWidthIn": n, "bitWidthOut": m, "groupSize": g},
+        "parity_ok": ok,
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved_tops,
+            "peak": VALU_PEAK_TOPS,
+            "unit": "TOP/s",
+            "frac": achieved_tops / VALU_PEAK_TOPS,
+            "traffic": None,
+            "kernel": "dcf_eval_g1_kernel<uint32_t>",
+            "kernel_ms": kernel_ms,
+            "ops_per_eval": n * OPS_PER_LEVEL,
+            "hbm_achieved_gbs": hbm_gbs,
+            "hbm_frac": hbm_gbs / HBM_PEAK_GBS,
+        },
+    }
+    if not a.no_cpu:
+        res["cpu_baseline"] = cpu_baseline(a.cpu_sample_log2)
+    print(json.dumps(res))
+    return 0 if ok else 1
+
+
+if __name__ == "__main__":
+    sys.exit(main())

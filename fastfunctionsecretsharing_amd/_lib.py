@@ -1,0 +1,106 @@
+"""Loader for the device library ``libfastfss_hip.so`` and its C ABI.
+
+The library is the product: every ``FastFss_cuda_*`` entry point runs HIP
+kernels for gfx950.  There is no fallback path; if the library is missing the
+import fails with an error naming the build command.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libfastfss_hip.so"
+
+# argument kinds: p = pointer, s = size_t, i = int, b = bool
+_EVAL = "pspspsps" + "i" + "sssss" + "ps"
+SIGNATURES = {
+    # name: (argument kinds, restype)
+    "FastFss_cuda_dcfKeyGen": ("pspspspsps" + "sssss" + "p", "int"),
+    "FastFss_cuda_dcfEval": (_EVAL + "p", "int"),
+    "FastFss_cuda_dpfKeyGen": ("pspspspsps" + "sssss" + "p", "int"),
+    "FastFss_cuda_dpfEval": (_EVAL + "p", "int"),
+    "FastFss_cuda_dpfEvalAll": (_EVAL + "p", "int"),
+    "FastFss_cuda_dpfEvalMulti": ("pspspsps" + "i" + "ps" + "sssss" + "ps" + "p", "int"),
+    "FastFss_cuda_grottoKeyGen": ("pspspsps" + "sss" + "p", "int"),
+    "FastFss_cuda_grottoEqEval": ("pspspsps" + "i" + "sss" + "ps" + "p", "int"),
+    "FastFss_cuda_grottoEval": ("pspspsps" + "b" + "i" + "sss" + "ps" + "p", "int"),
+    "FastFss_cuda_grottoMICEval": ("pspspsps" + "i" + "psps" + "sss" + "ps" + "p", "int"),
+    "FastFss_cuda_grottoIntervalLutEval": ("pspspspsps" + "i" + "pspsps" + "ssss" + "ps" + "p", "int"),
+    "FastFss_cuda_dcfMICKeyGen": ("pspspspspspsps" + "ssss" + "p", "int"),
+    "FastFss_cuda_dcfMICEval": ("pspspspsps" + "i" + "psps" + "ssss" + "ps" + "p", "int"),
+    "FastFss_cuda_dcfSplineKeyGen": ("pspspspspspsps" + "s" + "psps" + "sssss" + "p", "int"),
+    "FastFss_cuda_dcfSplineEval": ("pspspspspsps" + "i" + "psps" + "ssssss" + "ps" + "p", "int"),
+    "FastFss_cuda_otttKeyGen": ("psps" + "sss" + "p", "int"),
+    "FastFss_cuda_otttLutEval": ("pspspsps" + "i" + "ps" + "sss" + "p", "int"),
+    "FastFss_cuda_prngInit": ("", "ptr"),
+    "FastFss_cuda_prngRelease": ("p", None),
+    "FastFss_cuda_prngSetCurrentSeed": ("ppp", "int"),
+    "FastFss_cuda_prngGetCurrentSeed": ("ppp", "int"),
+    "FastFss_cuda_prngGen": ("ppsss" + "p", "int"),
+    "FastFss_cuda_setGridDim": ("i", "int"),
+    "FastFss_cuda_getGridDim": ("", "int"),
+    "FastFss_cuda_setFineGrainParallelGridDimThreshold": ("i", "int"),
+    "FastFss_cuda_getFineGrainParallelGridDimThreshold": ("", "int"),
+    "FastFss_dcfKeyZip": ("psps" + "sssss", "int"),
+    "FastFss_dcfKeyUnzip": ("psps" + "sssss", "int"),
+    "FastFss_dcfGetKeyDataSize": ("psssss", "int"),
+    "FastFss_dcfGetZippedKeyDataSize": ("psssss", "int"),
+    "FastFss_dcfGetCacheDataSize": ("psssss", "int"),
+    "FastFss_dpfKeyZip": ("psps" + "sssss", "int"),
+    "FastFss_dpfKeyUnzip": ("psps" + "sssss", "int"),
+    "FastFss_dpfGetKeyDataSize": ("psssss", "int"),
+    "FastFss_dpfGetZippedKeyDataSize": ("psssss", "int"),
+    "FastFss_dpfGetCacheDataSize": ("psss", "int"),
+    "FastFss_grottoKeyZip": ("psps" + "sss", "int"),
+    "FastFss_grottoKeyUnzip": ("psps" + "sss", "int"),
+    "FastFss_grottoGetKeyDataSize": ("psss", "int"),
+    "FastFss_grottoGetZippedKeyDataSize": ("psss", "int"),
+    "FastFss_grottoGetCacheDataSize": ("psss", "int"),
+    "FastFss_dcfMICKeyZip": ("psps" + "ssss", "int"),
+    "FastFss_dcfMICKeyUnzip": ("psps" + "ssss", "int"),
+    "FastFss_dcfMICGetCacheDataSize": ("pssss", "int"),
+    "FastFss_dcfMICGetKeyDataSize": ("pssss", "int"),
+    "FastFss_dcfMICGetZippedKeyDataSize": ("pssss", "int"),
+    "FastFss_dcfSplineKeyZip": ("psps" + "ssssss", "int"),
+    "FastFss_dcfSplineKeyUnzip": ("psps" + "ssssss", "int"),
+    "FastFss_dcfSplineGetCacheDataSize": ("pssssss", "int"),
+    "FastFss_dcfSplineGetKeyDataSize": ("pssssss", "int"),
+    "FastFss_dcfSplineGetZippedKeyDataSize": ("pssssss", "int"),
+    "FastFss_otttGetKeyDataSize": ("pss", "int"),
+}
+
+_KIND = {"p": ctypes.c_void_p, "s": ctypes.c_size_t, "i": ctypes.c_int, "b": ctypes.c_bool}
+_RES = {"int": ctypes.c_int, "ptr": ctypes.c_void_p, None: None}
+
+_lib = None
+
+
+class LibraryMissing(RuntimeError):
+    pass
+
+
+def load(path: Path | str | None = None):
+    """Load (once) and return the ctypes handle with argument types bound."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise LibraryMissing(
+            f"{p} is not built; run `python -m fastfunctionsecretsharing_amd.build` "
+            "(the MI355X kernels are the only implementation)")
+    lib = ctypes.CDLL(str(p))
+    for name, (kinds, res) in SIGNATURES.items():
+        fn = getattr(lib, name, None)
+        if fn is None:
+            continue
+        fn.argtypes = [_KIND[k] for k in kinds]
+        fn.restype = _RES[res]
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def exported_symbols(path: Path | str | None = None):
+    lib = load(path)
+    return {name for name in SIGNATURES if getattr(lib, name, None) is not None}

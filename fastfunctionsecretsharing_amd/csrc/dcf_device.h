@@ -1,0 +1,255 @@
+// DCF tree walks on the device (reference src/impl/dcf.h:187-491).
+//
+// Key layout (reference src/impl/dcf.h:139-167), N elements, n = bitWidthIn:
+//   [sCW: N x n x 16 bytes][per element: vCW[n*g] | tLCW | tRCW | lastCW[g]]
+// The sCW region is one 16n-byte run per element; the tail is an AoS record of
+// (n*g + 2 + g) group elements.
+//
+// Evaluation walks only the taken child at each level: the reference computes
+// both children (4 AES blocks) but reads only {sL, vL} or {sR, vR}
+// (src/impl/dcf.h:424,453-464), so two blocks with plaintexts (4x, 4x+1) and
+// (4x+2, 4x+3) give identical shares at half the AES work.
+#pragma once
+
+#include "fss_device.h"
+
+namespace fss {
+
+template <typename T>
+__device__ __forceinline__ T neg_if(T v, int party)
+{
+    return party ? (T)(0 - v) : v;
+}
+
+// One party's share for groupSize 1 (the hot path).  `s` is the raw 16-byte
+// seed; `x` the masked input; outBytes = ceil(bitWidthOut / 8).
+template <typename T>
+__device__ __forceinline__ T dcf_eval_g1(const AesCtx &c, const uint8_t *sCW, const T *tail, T x, uint32_t s[4],
+                                         int party, int n, int outBytes)
+{
+    s[0] &= ~1u;
+    uint32_t t   = (uint32_t)party;
+    x            = mod_bits<T>(x, n);
+    T        acc = 0;
+    const T  tl = tail[n], tr = tail[n + 1];
+    uint32_t cw[4];
+    load16(cw, sCW);
+    T vcw = tail[0];
+    for (int i = 0; i < n; i++)
+    {
+        const uint32_t xi = (uint32_t)(x >> (n - 1 - i)) & 1u;
+        uint32_t       cwn[4];
+        T              vcwn = 0;
+        if (i + 1 < n)
+        {
+            load16(cwn, sCW + 16 * (i + 1));
+            vcwn = tail[i + 1];
+        }
+        uint32_t blk[2][4];
+        blk[0][0] = 4u * xi, blk[0][1] = 0u, blk[0][2] = 4u * xi + 1u, blk[0][3] = 0u;
+        blk[1][0] = 4u * xi + 2u, blk[1][1] = 0u, blk[1][2] = 4u * xi + 3u, blk[1][3] = 0u;
+        aes_encrypt<2>(c, s, blk);
+        const uint32_t m  = 0u - t;
+        uint32_t       tn = blk[0][0] & 1u;
+        s[0]              = (blk[0][0] & ~1u) ^ (cw[0] & m);
+        s[1]              = blk[0][1] ^ (cw[1] & m);
+        s[2]              = blk[0][2] ^ (cw[2] & m);
+        s[3]              = blk[0][3] ^ (cw[3] & m);
+        tn ^= (uint32_t)(((xi ? tr : tl) >> i) & 1) & t;
+        T term = (T)((T)low_bytes(blk[1], outBytes) + (t ? vcw : (T)0));
+        acc    = (T)(acc + neg_if<T>(term, party));
+        t      = tn;
+#pragma unroll
+        for (int k = 0; k < 4; k++) cw[k] = cwn[k];
+        vcw = vcwn;
+    }
+    T term = (T)((T)low_bytes(s, outBytes) + (t ? tail[n + 2] : (T)0));
+    return (T)(acc + neg_if<T>(term, party));
+}
+
+// Share for group elements [j0, j0 + cnt) of a groupSize-g DCF; cnt <= R.
+// Used for g > 1 (R covers the group) and by the spline evaluator, which
+// needs only the coefficient ranges of one or two intervals.
+template <typename T, int R>
+__device__ __forceinline__ void dcf_eval_range(const AesCtx &c, T acc[R], const uint8_t *sCW, const T *tail, T x,
+                                               const uint32_t seed[4], int party, int n, int m, int g, int j0,
+                                               int cnt)
+{
+    uint32_t s[4] = {seed[0] & ~1u, seed[1], seed[2], seed[3]};
+    uint32_t t    = (uint32_t)party;
+    x             = mod_bits<T>(x, n);
+#pragma unroll
+    for (int r = 0; r < R; r++) acc[r] = 0;
+    const T tl = tail[(size_t)n * g], tr = tail[(size_t)n * g + 1];
+    for (int i = 0; i < n; i++)
+    {
+        const uint32_t xi = (uint32_t)(x >> (n - 1 - i)) & 1u;
+        uint32_t       cw[4];
+        load16(cw, sCW + 16 * i);
+        uint32_t blk[2][4];
+        blk[0][0] = 4u * xi, blk[0][1] = 0u, blk[0][2] = 4u * xi + 1u, blk[0][3] = 0u;
+        blk[1][0] = 4u * xi + 2u, blk[1][1] = 0u, blk[1][2] = 4u * xi + 3u, blk[1][3] = 0u;
+        aes_encrypt<2>(c, s, blk);
+        const uint32_t mk = 0u - t;
+        uint32_t       tn = blk[0][0] & 1u;
+        s[0]              = (blk[0][0] & ~1u) ^ (cw[0] & mk);
+        s[1]              = blk[0][1] ^ (cw[1] & mk);
+        s[2]              = blk[0][2] ^ (cw[2] & mk);
+        s[3]              = blk[0][3] ^ (cw[3] & mk);
+        tn ^= (uint32_t)(((xi ? tr : tl) >> i) & 1) & t;
+        ConvCtx cv;
+        cv.init(blk[1], g, m);
+        const T *vcw = tail + (size_t)i * g + j0;
+#pragma unroll
+        for (int r = 0; r < R; r++)
+        {
+            if (r < cnt)
+            {
+                T term = (T)((T)cv.at(c, j0 + r) + (t ? vcw[r] : (T)0));
+                acc[r] = (T)(acc[r] + neg_if<T>(term, party));
+            }
+        }
+        t = tn;
+    }
+    ConvCtx cv;
+    cv.init(s, g, m);
+    const T *last = tail + (size_t)n * g + 2 + j0;
+#pragma unroll
+    for (int r = 0; r < R; r++)
+    {
+        if (r < cnt)
+        {
+            T term = (T)((T)cv.at(c, j0 + r) + (t ? last[r] : (T)0));
+            acc[r] = (T)(acc[r] + neg_if<T>(term, party));
+        }
+    }
+}
+
+// Any groupSize: accumulate straight into the element's output slice.
+template <typename T>
+__device__ void dcf_eval_global(const AesCtx &c, T *out, const uint8_t *sCW, const T *tail, T x,
+                                const uint32_t seed[4], int party, int n, int m, int g)
+{
+    uint32_t s[4] = {seed[0] & ~1u, seed[1], seed[2], seed[3]};
+    uint32_t t    = (uint32_t)party;
+    x             = mod_bits<T>(x, n);
+    for (int j = 0; j < g; j++) out[j] = 0;
+    const T tl = tail[(size_t)n * g], tr = tail[(size_t)n * g + 1];
+    for (int i = 0; i < n; i++)
+    {
+        const uint32_t xi = (uint32_t)(x >> (n - 1 - i)) & 1u;
+        uint32_t       cw[4];
+        load16(cw, sCW + 16 * i);
+        uint32_t blk[2][4];
+        blk[0][0] = 4u * xi, blk[0][1] = 0u, blk[0][2] = 4u * xi + 1u, blk[0][3] = 0u;
+        blk[1][0] = 4u * xi + 2u, blk[1][1] = 0u, blk[1][2] = 4u * xi + 3u, blk[1][3] = 0u;
+        aes_encrypt<2>(c, s, blk);
+        const uint32_t mk = 0u - t;
+        uint32_t       tn = blk[0][0] & 1u;
+        s[0]              = (blk[0][0] & ~1u) ^ (cw[0] & mk);
+        s[1]              = blk[0][1] ^ (cw[1] & mk);
+        s[2]              = blk[0][2] ^ (cw[2] & mk);
+        s[3]              = blk[0][3] ^ (cw[3] & mk);
+        tn ^= (uint32_t)(((xi ? tr : tl) >> i) & 1) & t;
+        ConvCtx cv;
+        cv.init(blk[1], g, m);
+        const T *vcw = tail + (size_t)i * g;
+        for (int j = 0; j < g; j++)
+        {
+            T term = (T)((T)cv.at(c, j) + (t ? vcw[j] : (T)0));
+            out[j] = (T)(out[j] + neg_if<T>(term, party));
+        }
+        t = tn;
+    }
+    ConvCtx cv;
+    cv.init(s, g, m);
+    const T *last = tail + (size_t)n * g + 2;
+    for (int j = 0; j < g; j++)
+    {
+        T term = (T)((T)cv.at(c, j) + (t ? last[j] : (T)0));
+        out[j] = (T)(out[j] + neg_if<T>(term, party));
+    }
+}
+
+// Key generation for one element (reference src/impl/dcf.h:187-368).
+// `beta` == nullptr means beta = (1, 0, ...).  lastCW doubles as the running
+// vAlpha accumulator, as in the reference.
+template <typename T>
+__device__ void dcf_keygen_one(const AesCtx &c, uint8_t *sCW, T *tail, T alpha, const T *beta,
+                               const uint32_t seed0[4], const uint32_t seed1[4], int n, int m, int g)
+{
+    T *vcw  = tail;
+    T *last = tail + (size_t)n * g + 2;
+    for (int j = 0; j < g; j++) last[j] = 0;
+    uint32_t s0[4] = {seed0[0] & ~1u, seed0[1], seed0[2], seed0[3]};
+    uint32_t s1[4] = {seed1[0] & ~1u, seed1[1], seed1[2], seed1[3]};
+    uint32_t t0 = 0, t1 = 1;
+    T        tlBits = 0, trBits = 0;
+    for (int i = 0; i < n; i++)
+    {
+        uint32_t b0[4][4], b1[4][4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) prg_pt(b0[q], q), prg_pt(b1[q], q);
+        aes_encrypt<4>(c, s0, b0);
+        aes_encrypt<4>(c, s1, b1);
+        const uint32_t tL0 = b0[0][0] & 1u, tR0 = b0[2][0] & 1u, tL1 = b1[0][0] & 1u, tR1 = b1[2][0] & 1u;
+        b0[0][0] &= ~1u, b0[2][0] &= ~1u, b1[0][0] &= ~1u, b1[2][0] &= ~1u;
+        const int keep = (int)((alpha >> (n - 1 - i)) & 1);
+        const int lose = 1 - keep;
+        uint32_t  cw[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) cw[k] = (lose ? b0[2][k] ^ b1[2][k] : b0[0][k] ^ b1[0][k]);
+        store16(sCW + 16 * i, cw);
+
+        const T sign1 = t1 ? (T)-1 : (T)1;
+        ConvCtx cl1, cl0, ck1, ck0;
+        uint32_t vl0[4], vl1[4], vk0[4], vk1[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+        {
+            vl0[k] = lose ? b0[3][k] : b0[1][k];
+            vl1[k] = lose ? b1[3][k] : b1[1][k];
+            vk0[k] = keep ? b0[3][k] : b0[1][k];
+            vk1[k] = keep ? b1[3][k] : b1[1][k];
+        }
+        cl1.init(vl1, g, m), cl0.init(vl0, g, m), ck1.init(vk1, g, m), ck0.init(vk0, g, m);
+        for (int j = 0; j < g; j++)
+        {
+            T v = (T)((T)cl1.at(c, j) - (T)cl0.at(c, j) - last[j]);
+            v   = (T)(v * sign1);
+            if (lose == 0)
+            {
+                T b = beta ? beta[j] : (T)(j == 0 ? 1 : 0);
+                v   = (T)(v + (T)(sign1 * b));
+            }
+            vcw[(size_t)i * g + j] = v;
+            last[j]                = (T)(last[j] - (T)ck1.at(c, j) + (T)ck0.at(c, j) + (T)(sign1 * v));
+        }
+        const uint32_t tLCW = tL0 ^ tL1 ^ (uint32_t)keep ^ 1u;
+        const uint32_t tRCW = tR0 ^ tR1 ^ (uint32_t)keep;
+        tlBits |= (T)((T)tLCW << i);
+        trBits |= (T)((T)tRCW << i);
+        const uint32_t m0 = 0u - t0, m1 = 0u - t1;
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+        {
+            s0[k] = (keep ? b0[2][k] : b0[0][k]) ^ (cw[k] & m0);
+            s1[k] = (keep ? b1[2][k] : b1[0][k]) ^ (cw[k] & m1);
+        }
+        const uint32_t tk0 = keep ? tR0 : tL0, tk1 = keep ? tR1 : tL1, tkc = keep ? tRCW : tLCW;
+        t0 = tk0 ^ (t0 & tkc);
+        t1 = tk1 ^ (t1 & tkc);
+    }
+    tail[(size_t)n * g]     = tlBits;
+    tail[(size_t)n * g + 1] = trBits;
+    const T sign1 = t1 ? (T)-1 : (T)1;
+    ConvCtx c1, c0;
+    c1.init(s1, g, m), c0.init(s0, g, m);
+    for (int j = 0; j < g; j++)
+    {
+        T v     = (T)((T)(0 - last[j]) + (T)c1.at(c, j) - (T)c0.at(c, j));
+        last[j] = (T)(v * sign1);
+    }
+}
+
+} // namespace fss

@@ -1,0 +1,348 @@
+// Device-side building blocks for the FSS kernels on gfx950 (CDNA4).
+//
+// AES-128 with a per-lane key (the PRG of every tree level is AES keyed by the
+// node seed, reference src/impl/dcf.h:423-424 / src/impl/aes.h:466-469), written
+// as a T-table cipher whose two tables live in LDS:
+//
+//   * State words are little-endian columns of the FIPS-197 byte array
+//     (byte r of column c sits in bits 8r..8r+7 of word c), so 16-byte seeds
+//     and blocks load and store with plain dword accesses.
+//   * TE0[x] = 2S(x) | S(x)<<8 | S(x)<<16 | 3S(x)<<24 and TE1 = rotl8(TE0).
+//     Rows 2 and 3 use rotl16 of rows 0 and 1, so a column costs 4 lookups,
+//     2 three-way XORs (v_bitop3_b32) and one v_alignbit_b32.
+//   * Each table entry is replicated 64 times with lane-private copies: entry
+//     x for lane L is at byte x*256 + (L%64)*4, so a wave's 64 lookups hit 64
+//     distinct banks whatever the data.  A single v_perm_b32 builds the LDS
+//     address from the state byte and a per-lane constant.  TE0 occupies
+//     bytes [0, 64K), TE1 bytes [64K, 128K): 128 KB per workgroup.
+//   * The last round and the key schedule take S(x) out of the table entries
+//     (S sits in bytes 1-2 of TE0 and bytes 2-3 of TE1).
+//
+// The S-box is computed at compile time from the GF(2^8) inverse and the
+// affine map of FIPS-197 section 5.1.1.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace fss {
+
+// ---------------------------------------------------------------------------
+// compile-time AES tables
+// ---------------------------------------------------------------------------
+
+struct SboxTable
+{
+    uint8_t s[256];
+};
+
+constexpr uint8_t gf_xtime(uint8_t a)
+{
+    return (uint8_t)((a << 1) ^ ((a & 0x80) ? 0x1b : 0));
+}
+
+constexpr uint8_t gf_mul3(uint8_t a)
+{
+    return (uint8_t)(gf_xtime(a) ^ a);
+}
+
+constexpr SboxTable make_sbox()
+{
+    // powers of the generator 3 give log/antilog tables; inverse = 3^(255-log x)
+    uint8_t exp_t[256] = {};
+    uint8_t log_t[256] = {};
+    uint8_t p          = 1;
+    for (int i = 0; i < 255; i++)
+    {
+        exp_t[i] = p;
+        log_t[p] = (uint8_t)i;
+        p        = gf_mul3(p);
+    }
+    SboxTable t = {};
+    for (int x = 0; x < 256; x++)
+    {
+        uint8_t inv = x ? exp_t[(255 - log_t[x]) % 255] : 0;
+        uint8_t s   = inv;
+        uint8_t r   = inv;
+        for (int k = 0; k < 4; k++)
+        {
+            r = (uint8_t)((r << 1) | (r >> 7));
+            s ^= r;
+        }
+        t.s[x] = (uint8_t)(s ^ 0x63);
+    }
+    return t;
+}
+
+struct TeTable
+{
+    uint32_t v[256];
+};
+
+constexpr TeTable make_te0()
+{
+    SboxTable sb = make_sbox();
+    TeTable   t  = {};
+    for (int x = 0; x < 256; x++)
+    {
+        uint32_t s = sb.s[x];
+        t.v[x]     = (uint32_t)gf_xtime((uint8_t)s) | (s << 8) | (s << 16) | ((uint32_t)gf_mul3((uint8_t)s) << 24);
+    }
+    return t;
+}
+
+__constant__ static const TeTable kTe0 = make_te0();
+
+constexpr int kLdsTableWords = 2 * 256 * 64; // 128 KB
+
+// ---------------------------------------------------------------------------
+// bit helpers
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c)
+{
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+
+// (a | b) ^ c
+__device__ __forceinline__ uint32_t or_xor(uint32_t a, uint32_t b, uint32_t c)
+{
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x56);
+}
+
+__device__ __forceinline__ uint32_t rot16(uint32_t a)
+{
+    return __builtin_amdgcn_alignbit(a, a, 16);
+}
+
+__device__ __forceinline__ uint32_t perm(uint32_t hi, uint32_t lo, uint32_t sel)
+{
+    return __builtin_amdgcn_perm(hi, lo, sel);
+}
+
+// ---------------------------------------------------------------------------
+// LDS tables
+// ---------------------------------------------------------------------------
+
+// Fill both tables; every thread of the block takes part.  Call once per
+// workgroup, followed by __syncthreads().
+__device__ __forceinline__ void aes_fill_lds(uint32_t *lds)
+{
+    for (int w = threadIdx.x; w < 256 * 64; w += blockDim.x)
+    {
+        uint32_t v  = kTe0.v[w >> 6];
+        lds[w]      = v;
+        lds[w + 16384] = __builtin_amdgcn_alignbit(v, v, 24); // rotl8
+    }
+}
+
+struct AesCtx
+{
+    const char *base; // LDS table base
+    uint32_t    la;   // (lane % 64) * 4            -> TE0 copy of this lane
+    uint32_t    lb;   // (lane % 64) * 4 | 0x10000  -> TE1 copy of this lane
+};
+
+__device__ __forceinline__ AesCtx aes_ctx(const uint32_t *lds)
+{
+    AesCtx c;
+    c.base = (const char *)lds;
+    c.la   = (threadIdx.x & 63u) * 4u;
+    c.lb   = c.la | 0x10000u;
+    return c;
+}
+
+// LDS byte address of the table entry for byte k of s: [lane byte, s.byte k, table, 0]
+#define FSS_SEL(k) (0x0C020000u | ((4u + (k)) << 8))
+
+__device__ __forceinline__ uint32_t te0(const AesCtx &c, uint32_t s, int k)
+{
+    return *(const uint32_t *)(c.base + perm(s, c.la, FSS_SEL(k)));
+}
+
+__device__ __forceinline__ uint32_t te1(const AesCtx &c, uint32_t s, int k)
+{
+    return *(const uint32_t *)(c.base + perm(s, c.lb, FSS_SEL(k)));
+}
+
+// S-bytes of (a.b1, b.b1) into bytes 0,1 from two TE0 entries
+#define FSS_SEL_S01 0x0C0C0105u
+// S-bytes of (a.b2, b.b3) into bytes 2,3 from two TE1 entries
+#define FSS_SEL_S23 0x03060C0Cu
+
+// next AES-128 round key (little-endian words), round r in 1..10
+__device__ __forceinline__ void aes_next_key(const AesCtx &c, uint32_t k[4], uint32_t rcon)
+{
+    uint32_t lo = perm(te0(c, k[3], 1), te0(c, k[3], 2), FSS_SEL_S01);
+    uint32_t hi = perm(te1(c, k[3], 3), te1(c, k[3], 0), FSS_SEL_S23);
+    k[0]        = or_xor(lo, hi, k[0]) ^ rcon;
+    k[1] ^= k[0];
+    k[2] ^= k[1];
+    k[3] ^= k[2];
+}
+
+// one full round; h[c] = rot16(round key word c)
+__device__ __forceinline__ void aes_round(const AesCtx &c, uint32_t s[4], const uint32_t h[4])
+{
+    uint32_t n0 = xor3(te0(c, s[0], 0), te1(c, s[1], 1), rot16(xor3(te0(c, s[2], 2), te1(c, s[3], 3), h[0])));
+    uint32_t n1 = xor3(te0(c, s[1], 0), te1(c, s[2], 1), rot16(xor3(te0(c, s[3], 2), te1(c, s[0], 3), h[1])));
+    uint32_t n2 = xor3(te0(c, s[2], 0), te1(c, s[3], 1), rot16(xor3(te0(c, s[0], 2), te1(c, s[1], 3), h[2])));
+    uint32_t n3 = xor3(te0(c, s[3], 0), te1(c, s[0], 1), rot16(xor3(te0(c, s[1], 2), te1(c, s[2], 3), h[3])));
+    s[0] = n0, s[1] = n1, s[2] = n2, s[3] = n3;
+}
+
+__device__ __forceinline__ uint32_t aes_last_col(const AesCtx &c, uint32_t a, uint32_t b, uint32_t d, uint32_t e,
+                                                 uint32_t k)
+{
+    uint32_t lo = perm(te0(c, a, 0), te0(c, b, 1), FSS_SEL_S01);
+    uint32_t hi = perm(te1(c, d, 2), te1(c, e, 3), FSS_SEL_S23);
+    return or_xor(lo, hi, k);
+}
+
+__device__ __forceinline__ void aes_last(const AesCtx &c, uint32_t s[4], const uint32_t k[4])
+{
+    uint32_t n0 = aes_last_col(c, s[0], s[1], s[2], s[3], k[0]);
+    uint32_t n1 = aes_last_col(c, s[1], s[2], s[3], s[0], k[1]);
+    uint32_t n2 = aes_last_col(c, s[2], s[3], s[0], s[1], k[2]);
+    uint32_t n3 = aes_last_col(c, s[3], s[0], s[1], s[2], k[3]);
+    s[0] = n0, s[1] = n1, s[2] = n2, s[3] = n3;
+}
+
+// Encrypt NB blocks (in place) under the 16-byte key `key`, key schedule on the fly.
+template <int NB>
+__device__ __forceinline__ void aes_encrypt(const AesCtx &c, const uint32_t key[4], uint32_t st[NB][4])
+{
+    uint32_t k[4] = {key[0], key[1], key[2], key[3]};
+#pragma unroll
+    for (int b = 0; b < NB; b++)
+    {
+        st[b][0] ^= k[0], st[b][1] ^= k[1], st[b][2] ^= k[2], st[b][3] ^= k[3];
+    }
+    const uint32_t rcon[10] = {0x01, 0x02, 0x04, 0x08, 0x10, 0x20, 0x40, 0x80, 0x1b, 0x36};
+#pragma unroll
+    for (int r = 0; r < 9; r++)
+    {
+        aes_next_key(c, k, rcon[r]);
+        uint32_t h[4] = {rot16(k[0]), rot16(k[1]), rot16(k[2]), rot16(k[3])};
+#pragma unroll
+        for (int b = 0; b < NB; b++) aes_round(c, st[b], h);
+    }
+    aes_next_key(c, k, rcon[9]);
+#pragma unroll
+    for (int b = 0; b < NB; b++) aes_last(c, st[b], k);
+}
+
+// ---------------------------------------------------------------------------
+// PRG and value conversion
+// ---------------------------------------------------------------------------
+
+// Plaintext of counter block q: little-endian u64 pair (2q, 2q+1).
+__device__ __forceinline__ void prg_pt(uint32_t p[4], uint32_t q)
+{
+    p[0] = 2u * q, p[1] = 0u, p[2] = 2u * q + 1u, p[3] = 0u;
+}
+
+// Low `bytes` bytes (1..8) of a 16-byte block as an integer.
+__device__ __forceinline__ uint64_t low_bytes(const uint32_t v[4], int bytes)
+{
+    uint64_t w = (uint64_t)v[0] | ((uint64_t)v[1] << 32);
+    return bytes >= 8 ? w : (w & ((1ull << (8 * bytes)) - 1));
+}
+
+__device__ __forceinline__ uint32_t sel_word(const uint32_t v[4], int i)
+{
+    uint32_t r = v[0];
+    r          = (i == 1) ? v[1] : r;
+    r          = (i == 2) ? v[2] : r;
+    r          = (i == 3) ? v[3] : r;
+    return r;
+}
+
+__device__ __forceinline__ uint32_t byte_of(const uint32_t v[4], int idx)
+{
+    return (sel_word(v, idx >> 2) >> (8 * (idx & 3))) & 0xffu;
+}
+
+// AES_key(u64 0 || u64 idx): the conversion's refill block, kept out of line
+// so unrolled conversion loops stay small (the groupSize-1 paths never call it).
+__device__ __noinline__ static uint4 conv_refill(AesCtx c, uint32_t k0, uint32_t k1, uint32_t k2, uint32_t k3,
+                                                uint32_t idx)
+{
+    const uint32_t key[4] = {k0, k1, k2, k3};
+    uint32_t       blk[1][4] = {{0u, 0u, idx, 0u}};
+    aes_encrypt<1>(c, key, blk);
+    return make_uint4(blk[0][0], blk[0][1], blk[0][2], blk[0][3]);
+}
+
+// Group-element conversion of a 16-byte value (reference DcfConvertCtx,
+// src/impl/dcf.h:69-137).  Element j of a group of `count` elements of
+// `bytes` bytes: if bytes*count <= 16 it is bytes [j*bytes, (j+1)*bytes) of the
+// value; otherwise byte offset o = j*bytes+i comes from the block that the
+// reference refilled at the start of o's 16-byte window, AES_value(0, e) with
+// e = (16*floor(o/16)) / bytes the element being produced at that moment.
+// The struct caches the last block so sequential elements cost one AES block
+// per 16 bytes, like the reference.
+struct ConvCtx
+{
+    uint32_t key[4];
+    uint32_t buf[4];
+    int      bytes;
+    int      direct;
+    int      bufBlock; // window index held in buf (-1: none)
+
+    __device__ __forceinline__ void init(const uint32_t v[4], int count, int bitWidth)
+    {
+        bytes    = (bitWidth + 7) / 8;
+        direct   = bytes * count <= 16;
+        bufBlock = -1;
+#pragma unroll
+        for (int i = 0; i < 4; i++) key[i] = v[i], buf[i] = v[i];
+    }
+
+    __device__ __forceinline__ uint64_t at(const AesCtx &c, int j)
+    {
+        uint64_t r = 0;
+        for (int i = 0; i < bytes; i++)
+        {
+            int o = j * bytes + i;
+            if (!direct)
+            {
+                int win = o >> 4;
+                if (win != bufBlock)
+                {
+                    uint4 b  = conv_refill(c, key[0], key[1], key[2], key[3], (uint32_t)((win * 16) / bytes));
+                    buf[0]   = b.x, buf[1] = b.y, buf[2] = b.z, buf[3] = b.w;
+                    bufBlock = win;
+                }
+            }
+            if (i < 8) r |= (uint64_t)byte_of(buf, o & 15) << (8 * i);
+        }
+        return r;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// memory helpers
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ void load16(uint32_t d[4], const uint8_t *p)
+{
+    uint2 a = *(const uint2 *)p;
+    uint2 b = *(const uint2 *)(p + 8);
+    d[0] = a.x, d[1] = a.y, d[2] = b.x, d[3] = b.y;
+}
+
+__device__ __forceinline__ void store16(uint8_t *p, const uint32_t d[4])
+{
+    *(uint2 *)p       = make_uint2(d[0], d[1]);
+    *(uint2 *)(p + 8) = make_uint2(d[2], d[3]);
+}
+
+template <typename T>
+__device__ __forceinline__ T mod_bits(T x, int bw)
+{
+    return bw >= (int)(8 * sizeof(T)) ? x : (T)(x & (T)(((T)1 << bw) - 1));
+}
+
+} // namespace fss

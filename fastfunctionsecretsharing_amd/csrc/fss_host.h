@@ -1,0 +1,114 @@
+// Host-side helpers shared by the C-ABI entry points: error codes, element
+// type dispatch, stream resolution, persistent-grid sizing and size formulas.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "FastFss/errors.h"
+
+namespace fss {
+
+// Launches use one 1024-thread workgroup per CU: the AES tables take 128 KB
+// of the CU's 160 KB LDS, so a second workgroup would not fit anyway.
+constexpr int kBlock = 1024;
+
+int device_cu_count(); // cached per device
+
+inline hipStream_t resolve_stream(void *streamPtr)
+{
+    // nullptr selects the per-thread default stream, as the reference does
+    // (src/kernel/parallel_execute.h:164-168)
+    return streamPtr ? *(hipStream_t *)streamPtr : hipStreamPerThread;
+}
+
+inline int grid_for(size_t work, int wavesPerItemBlock = 1)
+{
+    size_t blocks = (work + kBlock - 1) / kBlock;
+    size_t cap    = (size_t)device_cu_count() * (size_t)wavesPerItemBlock;
+    if (blocks > cap) blocks = cap;
+    if (blocks == 0) blocks = 1;
+    return (int)blocks;
+}
+
+inline int launch_status()
+{
+    return hipPeekAtLastError() == hipSuccess ? FAST_FSS_SUCCESS : FAST_FSS_RUNTIME_ERROR;
+}
+
+inline bool elem_size_ok(size_t es)
+{
+    return es == 1 || es == 2 || es == 4 || es == 8;
+}
+
+inline bool aligned(const void *p, size_t a)
+{
+    return ((uintptr_t)p % a) == 0;
+}
+
+// ---- key / cache sizes (reference src/impl/<prim>.h) ----
+
+// src/impl/dcf.h:38-50
+__host__ __device__ inline size_t dcf_key_size(size_t n, size_t g, size_t es, size_t N)
+{
+    return N * (16 * n + es * n * g + es + es + es * g);
+}
+// src/impl/dcf.h:61-67
+__host__ __device__ inline size_t dcf_cache_size(size_t n, size_t g, size_t es, size_t N)
+{
+    return N * (16 * n) + N * es * n * g;
+}
+// src/impl/dpf.h:35-61
+__host__ __device__ inline size_t dpf_key_size(size_t n, size_t g, size_t es, size_t N)
+{
+    return N * (16 * n + es + es + es * g);
+}
+__host__ __device__ inline size_t dpf_cache_size(size_t n, size_t N)
+{
+    return N * (16 * n);
+}
+// src/impl/grotto.h:37-44,81-85
+__host__ __device__ inline size_t grotto_key_size(size_t n, size_t es, size_t N)
+{
+    return N * (16 * (n - 6) + 8 + es + es);
+}
+__host__ __device__ inline size_t grotto_cache_size(size_t n, size_t N)
+{
+    return N * (16 * (n - 6) + (n - 6));
+}
+// src/impl/spline.h:27-48
+__host__ __device__ inline size_t spline_cache_size(size_t n, size_t intervals, size_t degree, size_t es, size_t N)
+{
+    size_t g = intervals * (degree + 1);
+    return dcf_cache_size(n, g, es, N) + N * es * (2 * g + degree + 1);
+}
+
+} // namespace fss
+
+// Dispatch a runtime element size to a fixed-width unsigned type `T`.
+#define FSS_DISPATCH(ES, ...)                \
+    switch (ES)                              \
+    {                                        \
+        case 1: {                            \
+            using T = uint8_t;               \
+            __VA_ARGS__;                     \
+        }                                    \
+        break;                               \
+        case 2: {                            \
+            using T = uint16_t;              \
+            __VA_ARGS__;                     \
+        }                                    \
+        break;                               \
+        case 4: {                            \
+            using T = uint32_t;              \
+            __VA_ARGS__;                     \
+        }                                    \
+        break;                               \
+        case 8: {                            \
+            using T = uint64_t;              \
+            __VA_ARGS__;                     \
+        }                                    \
+        break;                               \
+        default: return FAST_FSS_INVALID_ELEMENT_SIZE_ERROR; \
+    }

@@ -1,0 +1,431 @@
+// DCF-based multiple-interval containment (MIC) and piecewise-polynomial
+// spline on MI355X (reference src/impl/mic.h, src/impl/spline.h,
+// src/kernel/mic.h, src/kernel/spline.h).
+//
+// Both evaluate the element's single DCF key at several points.  The spline
+// needs, at each point, only the coefficient slots of the interval(s) that
+// point bounds, so each point is walked once and only those group elements
+// (and the AES conversion blocks that hold them) are produced; the values are
+// identical to the reference's full-group evaluation.
+#include "FastFss/cuda/mic.h"
+#include "FastFss/cuda/spline.h"
+#include "dcf_device.h"
+#include "fss_host.h"
+
+namespace fss {
+
+#define FSS_KERNEL_PROLOGUE                 \
+    __shared__ uint32_t lds[kLdsTableWords]; \
+    aes_fill_lds(lds);                      \
+    __syncthreads();                        \
+    const AesCtx c = aes_ctx(lds);
+
+#define FSS_GRID_LOOP(i, count)                                                             \
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < (count); \
+         i += (size_t)gridDim.x * blockDim.x)
+
+// ------------------------------------------------------------------ MIC ----
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void mic_keygen_kernel(uint8_t *key, T *z, const T *alpha, const uint8_t *seed0,
+                                                            const uint8_t *seed1, const T *left, const T *right,
+                                                            size_t I, int n, int m, size_t N)
+{
+    FSS_KERNEL_PROLOGUE
+    T *tails = (T *)(key + N * 16 * (size_t)n);
+    const T MAX = mod_bits<T>((T)~(T)0, n);
+    FSS_GRID_LOOP(e, N)
+    {
+        uint32_t a[4], b[4];
+        load16(a, seed0 + 16 * e);
+        load16(b, seed1 + 16 * e);
+        const T al  = alpha[e];
+        const T one = 1;
+        // DCF on alpha - 1 with beta = 1 (src/impl/mic.h:22)
+        dcf_keygen_one<T>(c, key + e * 16 * (size_t)n, tails + e * (size_t)(n + 3), (T)(al - 1), &one, a, b, n, m,
+                          1);
+        for (size_t i = 0; i < I; i++)
+        {
+            const T qP = mod_bits<T>((T)(right[i] + 1), n);
+            const T aP = mod_bits<T>((T)(left[i] + al), n);
+            const T aQ = mod_bits<T>((T)(right[i] + al), n);
+            const T aQp = mod_bits<T>((T)(right[i] + 1 + al), n);
+            const T p   = mod_bits<T>(left[i], n);
+            const int zv = (int)(aP > aQ) - (int)(aP > p) + (int)(aQp > qP) + (int)(aQ == MAX);
+            z[e * I + i] = (T)zv;
+        }
+    }
+}
+
+// Interval shares [first, first+cnt) of one element (src/impl/mic.h:43-109).
+template <typename T>
+__device__ __forceinline__ void mic_eval_range(const AesCtx &c, T *out, const T *zs, const uint8_t *sCW,
+                                               const T *tail, T x, const uint32_t seed[4], int party, const T *left,
+                                               const T *right, size_t first, size_t cnt, int n, int outBytes)
+{
+    x    = mod_bits<T>(x, n);
+    T sp = 0, sq = 0;
+    for (size_t i = first; i < first + cnt; i++)
+    {
+        const T qP  = mod_bits<T>((T)(right[i] + 1), n);
+        const T xP  = mod_bits<T>((T)(x - 1 - left[i]), n);
+        const T xQp = mod_bits<T>((T)(x - 1 - (T)(right[i] + 1)), n);
+        const T p   = mod_bits<T>(left[i], n);
+        uint32_t s[4];
+        if (i > first && left[i] == mod_bits<T>((T)(right[i - 1] + 1), n))
+            sp = sq;
+        else
+        {
+            s[0] = seed[0], s[1] = seed[1], s[2] = seed[2], s[3] = seed[3];
+            sp   = dcf_eval_g1<T>(c, sCW, tail, xP, s, party, n, outBytes);
+        }
+        s[0] = seed[0], s[1] = seed[1], s[2] = seed[2], s[3] = seed[3];
+        sq  = dcf_eval_g1<T>(c, sCW, tail, xQp, s, party, n, outBytes);
+        T o = (T)(sq - sp + zs[i]);
+        if (party == 1) o = (T)(o + (T)((int)(x > p) - (int)(x > qP)));
+        out[i] = o;
+    }
+}
+
+// Work item = (element, block of intervals): small element counts still fill
+// the GPU; the first interval of a block is evaluated without the reuse of
+// its predecessor, which gives the same value (src/kernel/mic.h:189-272).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void mic_eval_kernel(T *out, const T *x, const uint8_t *key, const T *z,
+                                                          const uint8_t *seed, int party, const T *left,
+                                                          const T *right, size_t I, size_t blocksPerElem, int n,
+                                                          int outBytes, size_t N)
+{
+    FSS_KERNEL_PROLOGUE
+    const T     *tails = (const T *)(key + N * 16 * (size_t)n);
+    const size_t span  = (I + blocksPerElem - 1) / blocksPerElem;
+    FSS_GRID_LOOP(w, N * blocksPerElem)
+    {
+        const size_t e = w / blocksPerElem, blk = w % blocksPerElem;
+        const size_t first = blk * span;
+        if (first >= I) continue;
+        const size_t cnt = (first + span < I) ? span : (I - first);
+        uint32_t     s[4];
+        load16(s, seed + 16 * e);
+        mic_eval_range<T>(c, out + e * I, z + e * I, key + e * 16 * (size_t)n, tails + e * (size_t)(n + 3), x[e], s,
+                          party, left, right, first, cnt, n, outBytes);
+    }
+}
+
+// --------------------------------------------------------------- spline ----
+
+// coefficients of f(x - shift) mod 2^m (src/impl/spline.h:89-117)
+template <typename T>
+__device__ void spline_shift(T *outc, const T *coef, T shift, int d, int m)
+{
+    const T neg = mod_bits<T>((T)((T)0 - shift), m);
+    for (int i = 0; i <= d; i++) outc[i] = 0;
+    for (int i = d + 1; i > 0; --i)
+    {
+        for (int j = d; j > 0; --j) outc[j] = (T)(outc[j - 1] + (T)(outc[j] * neg));
+        outc[0] = (T)((T)(outc[0] * neg) + coef[i - 1]);
+    }
+    for (int i = 0; i <= d; i++) outc[i] = mod_bits<T>(outc[i], m);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void spline_keygen_kernel(uint8_t *key, T *eOut, T *betaOut, const T *alpha,
+                                                               const uint8_t *seed0, const uint8_t *seed1,
+                                                               const T *coef, int d, const T *left, const T *right,
+                                                               size_t I, int n, int m, size_t N)
+{
+    FSS_KERNEL_PROLOGUE
+    const size_t C = (size_t)d + 1, g = I * C;
+    T           *tails = (T *)(key + N * 16 * (size_t)n);
+    const T      maxv  = mod_bits<T>((T)~(T)0, n);
+    FSS_GRID_LOOP(e, N)
+    {
+        const T al   = alpha[e];
+        T      *beta = betaOut + e * g;
+        for (size_t q = 0; q < I; q++) spline_shift<T>(beta + q * C, coef + q * C, al, d, m);
+        uint32_t a[4], b[4];
+        load16(a, seed0 + 16 * e);
+        load16(b, seed1 + 16 * e);
+        dcf_keygen_one<T>(c, key + e * 16 * (size_t)n, tails + e * (size_t)(n * g + 2 + g), (T)(al - 1), beta, a, b,
+                          n, m, (int)g);
+        for (size_t q = 0; q < I; q++)
+        {
+            const T L = mod_bits<T>(left[q], n), R = mod_bits<T>(right[q], n);
+            const T Rp = mod_bits<T>((T)(R + 1), n);
+            const T aL = mod_bits<T>((T)(L + al), n), aR = mod_bits<T>((T)(R + al), n);
+            const T aRp = mod_bits<T>((T)(Rp + al), n);
+            const T cr  = (T)((int)(aL > aR) - (int)(aL > L) + (int)(aRp > Rp) + (int)(aR == maxv));
+            for (size_t j = 0; j < C; j++) eOut[e * g + q * C + j] = mod_bits<T>((T)(cr * beta[q * C + j]), m);
+        }
+    }
+}
+
+// Fast path: degree <= 7.  Each evaluation point x-1-L_0, x-1-(R_q+1) is walked
+// once; the walk at x-1-(R_q+1) also yields interval q+1's "current" slots when
+// intervals q and q+1 are adjacent (the reference copies the whole vector,
+// src/impl/spline.h:214-223).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void spline_eval_kernel(T *out, const T *x, const uint8_t *key, const T *sE,
+                                                             const T *sBeta, const uint8_t *seed, int party,
+                                                             const T *left, const T *right, size_t I, int d, int n,
+                                                             int m, size_t N)
+{
+    FSS_KERNEL_PROLOGUE
+    const int    C     = d + 1;
+    const size_t g     = I * (size_t)C;
+    const T     *tails = (const T *)(key + N * 16 * (size_t)n);
+    FSS_GRID_LOOP(e, N)
+    {
+        uint32_t s[4];
+        load16(s, seed + 16 * e);
+        const uint8_t *sc   = key + e * 16 * (size_t)n;
+        const T       *tail = tails + e * (n * g + 2 + g);
+        const T        xv   = mod_bits<T>(x[e], n);
+        const T       *be = sBeta + e * g, *ee = sE + e * g;
+        T              w[8], carry[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) w[j] = 0, carry[j] = 0;
+        for (size_t q = 0; q < I; q++)
+        {
+            const T L  = mod_bits<T>(left[q], n), R = mod_bits<T>(right[q], n);
+            const T Rp = mod_bits<T>((T)(R + 1), n);
+            T       cur[8];
+            if (q > 0 && mod_bits<T>((T)(right[q - 1] + 1), n) == L)
+            {
+#pragma unroll
+                for (int j = 0; j < 8; j++) cur[j] = carry[j];
+            }
+            else
+            {
+                T acc[16];
+                dcf_eval_range<T, 16>(c, acc, sc, tail, (T)(xv - 1 - L), s, party, n, m, (int)g, (int)(q * C), C);
+#pragma unroll
+                for (int j = 0; j < 8; j++) cur[j] = acc[j];
+            }
+            const bool link = (q + 1 < I) && mod_bits<T>((T)(right[q] + 1), n) == mod_bits<T>(left[q + 1], n);
+            T          acc[16];
+            dcf_eval_range<T, 16>(c, acc, sc, tail, (T)(xv - 1 - Rp), s, party, n, m, (int)g, (int)(q * C),
+                                  link ? 2 * C : C);
+            const T cx = (T)((int)(xv > L) - (int)(xv > Rp));
+#pragma unroll
+            for (int j = 0; j < 8; j++)
+            {
+                if (j < C)
+                    w[j] = (T)(w[j] + (T)(cx * be[q * C + j]) - cur[j] + acc[j] + ee[q * C + j]);
+            }
+            // slots C..2C-1 of acc hold interval q+1's values at this point
+#pragma unroll
+            for (int j = 0; j < 8; j++)
+            {
+                T v = 0;
+#pragma unroll
+                for (int r = 0; r < 16; r++) v = (r == j + C) ? acc[r] : v;
+                carry[j] = v;
+            }
+        }
+        T r = 0, pw = 1;
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+        {
+            if (j < C)
+            {
+                r  = (T)(r + (T)(w[j] * pw));
+                pw = (T)(pw * xv);
+            }
+        }
+        out[e] = mod_bits<T>(r, m);
+    }
+}
+
+// Any degree: the reference's per-element workspace inside the caller's
+// cache holds the two group vectors (src/impl/spline.h:64-87).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void spline_eval_wide_kernel(T *out, const T *x, const uint8_t *key, const T *sE,
+                                                                  const T *sBeta, const uint8_t *seed, int party,
+                                                                  const T *left, const T *right, size_t I, int d,
+                                                                  int n, int m, size_t N, uint8_t *cache)
+{
+    FSS_KERNEL_PROLOGUE
+    const size_t C = (size_t)d + 1, g = I * C;
+    const T     *tails = (const T *)(key + N * 16 * (size_t)n);
+    uint8_t     *ws    = cache + dcf_cache_size((size_t)n, g, sizeof(T), N);
+    FSS_GRID_LOOP(e, N)
+    {
+        uint32_t s[4];
+        load16(s, seed + 16 * e);
+        const uint8_t *sc   = key + e * 16 * (size_t)n;
+        const T       *tail = tails + e * (n * g + 2 + g);
+        T             *cur  = (T *)(ws + e * sizeof(T) * (2 * g + C));
+        T             *nxt  = cur + g;
+        T             *coef = nxt + g;
+        const T        xv   = mod_bits<T>(x[e], n);
+        for (size_t j = 0; j < C; j++) coef[j] = 0;
+        for (size_t q = 0; q < I; q++)
+        {
+            const T L  = mod_bits<T>(left[q], n), R = mod_bits<T>(right[q], n);
+            const T Rp = mod_bits<T>((T)(R + 1), n);
+            if (q > 0 && mod_bits<T>((T)(right[q - 1] + 1), n) == L)
+                for (size_t j = 0; j < g; j++) cur[j] = nxt[j];
+            else
+                dcf_eval_global<T>(c, cur, sc, tail, (T)(xv - 1 - L), s, party, n, m, (int)g);
+            dcf_eval_global<T>(c, nxt, sc, tail, (T)(xv - 1 - Rp), s, party, n, m, (int)g);
+            const T cx = (T)((int)(xv > L) - (int)(xv > Rp));
+            for (size_t j = 0; j < C; j++)
+                coef[j] = (T)(coef[j] + (T)(cx * sBeta[e * g + q * C + j]) - cur[q * C + j] + nxt[q * C + j] +
+                              sE[e * g + q * C + j]);
+        }
+        T r = 0, pw = 1;
+        for (size_t j = 0; j < C; j++)
+        {
+            r  = (T)(r + (T)(coef[j] * pw));
+            pw = (T)(pw * xv);
+        }
+        out[e] = mod_bits<T>(r, m);
+    }
+}
+
+} // namespace fss
+
+using namespace fss;
+
+extern "C" FAST_FSS_API int FastFss_cuda_dcfMICKeyGen(void *key, size_t keyDataSize, void *z, size_t zDataSize,
+                                                      const void *alpha, size_t alphaDataSize, const void *seed0,
+                                                      size_t seedDataSize0, const void *seed1, size_t seedDataSize1,
+                                                      const void *leftEndpoints, size_t leftEndpointsDataSize,
+                                                      const void *rightEndpoints, size_t rightEndpointsDataSize,
+                                                      size_t bitWidthIn, size_t bitWidthOut, size_t elementSize,
+                                                      size_t elementNum, void *cudaStreamPtr)
+{
+    const size_t n = bitWidthIn, m = bitWidthOut, es = elementSize, N = elementNum;
+    if (!elem_size_ok(es)) return FAST_FSS_INVALID_ELEMENT_SIZE_ERROR;
+    const size_t I = leftEndpointsDataSize / es;
+    if (!(0 < n && n <= es * 8)) return FAST_FSS_INVALID_BITWIDTH_ERROR;
+    if (!(0 < m && m <= es * 8)) return FAST_FSS_INVALID_BITWIDTH_ERROR;
+    if (keyDataSize != dcf_key_size(n, 1, es, N)) return FAST_FSS_INVALID_KEY_DATA_SIZE_ERROR;
+    if (zDataSize != N * es * I) return FAST_FSS_INVALID_Z_DATA_SIZE_ERROR;
+    if (alphaDataSize != N * es) return FAST_FSS_INVALID_ALPHA_DATA_SIZE_ERROR;
+    if (seedDataSize0 != N * 16 || seedDataSize1 != N * 16) return FAST_FSS_INVALID_SEED_DATA_SIZE_ERROR;
+    if (leftEndpointsDataSize != I * es || rightEndpointsDataSize != I * es)
+        return FAST_FSS_INVALID_BOUNDARY_DATA_SIZE_ERROR;
+    if (N == 0) return FAST_FSS_SUCCESS;
+    if (!aligned(key, 8) || !aligned(seed0, 8) || !aligned(seed1, 8)) return FAST_FSS_RUNTIME_ERROR;
+    hipStream_t st = resolve_stream(cudaStreamPtr);
+    FSS_DISPATCH(es, mic_keygen_kernel<T><<<grid_for(N), kBlock, 0, st>>>(
+                         (uint8_t *)key, (T *)z, (const T *)alpha, (const uint8_t *)seed0, (const uint8_t *)seed1,
+                         (const T *)leftEndpoints, (const T *)rightEndpoints, I, (int)n, (int)m, N));
+    return launch_status();
+}
+
+extern "C" FAST_FSS_API int FastFss_cuda_dcfMICEval(void *sharedOut, size_t sharedOutDataSize, const void *maskedX,
+                                                    size_t maskedXDataSize, const void *key, size_t keyDataSize,
+                                                    const void *sharedZ, size_t sharedZDataSize, const void *seed,
+                                                    size_t seedDataSize, int partyId, const void *leftEndpoints,
+                                                    size_t leftEndpointsDataSize, const void *rightEndpoints,
+                                                    size_t rightEndpointsDataSize, size_t bitWidthIn,
+                                                    size_t bitWidthOut, size_t elementSize, size_t elementNum,
+                                                    void *cache, size_t cacheDataSize, void *cudaStreamPtr)
+{
+    (void)cache;
+    const size_t n = bitWidthIn, m = bitWidthOut, es = elementSize, N = elementNum;
+    if (!elem_size_ok(es)) return FAST_FSS_INVALID_ELEMENT_SIZE_ERROR;
+    const size_t I = leftEndpointsDataSize / es;
+    if (!(0 < n && n <= es * 8)) return FAST_FSS_INVALID_BITWIDTH_ERROR;
+    if (!(0 < m && m <= es * 8)) return FAST_FSS_INVALID_BITWIDTH_ERROR;
+    if (partyId != 0 && partyId != 1) return FAST_FSS_INVALID_PARTY_ID_ERROR;
+    if (keyDataSize != dcf_key_size(n, 1, es, N)) return FAST_FSS_INVALID_KEY_DATA_SIZE_ERROR;
+    if (sharedOutDataSize != N * es * I) return FAST_FSS_INVALID_SHARED_OUT_DATA_SIZE_ERROR;
+    if (maskedXDataSize != N * es) return FAST_FSS_INVALID_MASKED_X_DATA_SIZE_ERROR;
+    if (sharedZDataSize != N * es * I) return FAST_FSS_INVALID_Z_DATA_SIZE_ERROR;
+    if (seedDataSize != N * 16) return FAST_FSS_INVALID_SEED_DATA_SIZE_ERROR;
+    if (leftEndpointsDataSize != I * es || rightEndpointsDataSize != I * es)
+        return FAST_FSS_INVALID_BOUNDARY_DATA_SIZE_ERROR;
+    if (cacheDataSize != 0 && cacheDataSize != dcf_cache_size(n, 1, es, N)) return FAST_FSS_INVALID_CACHE_DATA_SIZE_ERROR;
+    if (N == 0 || I == 0) return FAST_FSS_SUCCESS;
+    if (!aligned(key, 8) || !aligned(seed, 8)) return FAST_FSS_RUNTIME_ERROR;
+    hipStream_t st = resolve_stream(cudaStreamPtr);
+    // split each element's intervals until the launch has a lane per CU slot
+    const size_t lanes = (size_t)device_cu_count() * kBlock;
+    size_t       bpe   = (lanes + N - 1) / N;
+    if (bpe < 1) bpe = 1;
+    if (bpe > I) bpe = I;
+    FSS_DISPATCH(es, mic_eval_kernel<T><<<grid_for(N * bpe), kBlock, 0, st>>>(
+                         (T *)sharedOut, (const T *)maskedX, (const uint8_t *)key, (const T *)sharedZ,
+                         (const uint8_t *)seed, partyId, (const T *)leftEndpoints, (const T *)rightEndpoints, I, bpe,
+                         (int)n, (int)((m + 7) / 8), N));
+    return launch_status();
+}
+
+extern "C" FAST_FSS_API int FastFss_cuda_dcfSplineKeyGen(
+    void *key, size_t keyDataSize, void *e, size_t eDataSize, void *beta, size_t betaDataSize, const void *alpha,
+    size_t alphaDataSize, const void *seed0, size_t seedDataSize0, const void *seed1, size_t seedDataSize1,
+    const void *coefficients, size_t coefficientsDataSize, size_t degree, const void *leftEndpoints,
+    size_t leftEndpointsDataSize, const void *rightEndpoints, size_t rightEndpointsDataSize, size_t intervalNum,
+    size_t bitWidthIn, size_t bitWidthOut, size_t elementSize, size_t elementNum, void *cudaStreamPtr)
+{
+    const size_t n = bitWidthIn, m = bitWidthOut, es = elementSize, N = elementNum, I = intervalNum;
+    if (!elem_size_ok(es)) return FAST_FSS_INVALID_ELEMENT_SIZE_ERROR;
+    const size_t C = degree + 1, g = I * C;
+    if (!(0 < n && n <= es * 8)) return FAST_FSS_INVALID_BITWIDTH_ERROR;
+    if (!(0 < m && m <= es * 8)) return FAST_FSS_INVALID_BITWIDTH_ERROR;
+    if (I == 0) return FAST_FSS_INVALID_BOUNDARY_DATA_SIZE_ERROR;
+    if (keyDataSize != dcf_key_size(n, g, es, N)) return FAST_FSS_INVALID_KEY_DATA_SIZE_ERROR;
+    if (eDataSize != N * es * g) return FAST_FSS_INVALID_Z_DATA_SIZE_ERROR;
+    if (betaDataSize != N * es * g) return FAST_FSS_INVALID_BETA_DATA_SIZE_ERROR;
+    if (alphaDataSize != N * es) return FAST_FSS_INVALID_ALPHA_DATA_SIZE_ERROR;
+    if (seedDataSize0 != N * 16 || seedDataSize1 != N * 16) return FAST_FSS_INVALID_SEED_DATA_SIZE_ERROR;
+    if (leftEndpointsDataSize != I * es) return FAST_FSS_INVALID_BOUNDARY_DATA_SIZE_ERROR;
+    if (rightEndpointsDataSize != I * es) return FAST_FSS_INVALID_BOUNDARY_DATA_SIZE_ERROR;
+    if (coefficientsDataSize != I * C * es) return FAST_FSS_INVALID_BETA_DATA_SIZE_ERROR;
+    if (N == 0) return FAST_FSS_SUCCESS;
+    if (!aligned(key, 8) || !aligned(seed0, 8) || !aligned(seed1, 8)) return FAST_FSS_RUNTIME_ERROR;
+    hipStream_t st = resolve_stream(cudaStreamPtr);
+    FSS_DISPATCH(es, spline_keygen_kernel<T><<<grid_for(N), kBlock, 0, st>>>(
+                         (uint8_t *)key, (T *)e, (T *)beta, (const T *)alpha, (const uint8_t *)seed0,
+                         (const uint8_t *)seed1, (const T *)coefficients, (int)degree, (const T *)leftEndpoints,
+                         (const T *)rightEndpoints, I, (int)n, (int)m, N));
+    return launch_status();
+}
+
+extern "C" FAST_FSS_API int FastFss_cuda_dcfSplineEval(
+    void *sharedOut, size_t sharedOutDataSize, const void *maskedX, size_t maskedXDataSize, const void *key,
+    size_t keyDataSize, const void *sharedE, size_t sharedEDataSize, const void *sharedBeta, size_t sharedBetaDataSize,
+    const void *seed, size_t seedDataSize, int partyId, const void *leftEndpoints, size_t leftEndpointsDataSize,
+    const void *rightEndpoints, size_t rightEndpointsDataSize, size_t intervalNum, size_t degree, size_t bitWidthIn,
+    size_t bitWidthOut, size_t elementSize, size_t elementNum, void *cache, size_t cacheDataSize, void *cudaStreamPtr)
+{
+    const size_t n = bitWidthIn, m = bitWidthOut, es = elementSize, N = elementNum, I = intervalNum;
+    if (!elem_size_ok(es)) return FAST_FSS_INVALID_ELEMENT_SIZE_ERROR;
+    const size_t C = degree + 1, g = I * C;
+    if (!(0 < n && n <= es * 8)) return FAST_FSS_INVALID_BITWIDTH_ERROR;
+    if (!(0 < m && m <= es * 8)) return FAST_FSS_INVALID_BITWIDTH_ERROR;
+    if (partyId != 0 && partyId != 1) return FAST_FSS_INVALID_PARTY_ID_ERROR;
+    if (I == 0) return FAST_FSS_INVALID_BOUNDARY_DATA_SIZE_ERROR;
+    if (keyDataSize != dcf_key_size(n, g, es, N)) return FAST_FSS_INVALID_KEY_DATA_SIZE_ERROR;
+    if (sharedOutDataSize != N * es) return FAST_FSS_INVALID_SHARED_OUT_DATA_SIZE_ERROR;
+    if (maskedXDataSize != N * es) return FAST_FSS_INVALID_MASKED_X_DATA_SIZE_ERROR;
+    if (sharedEDataSize != N * es * g) return FAST_FSS_INVALID_Z_DATA_SIZE_ERROR;
+    if (sharedBetaDataSize != N * es * g) return FAST_FSS_INVALID_BETA_DATA_SIZE_ERROR;
+    if (seedDataSize != N * 16) return FAST_FSS_INVALID_SEED_DATA_SIZE_ERROR;
+    if (leftEndpointsDataSize != I * es) return FAST_FSS_INVALID_BOUNDARY_DATA_SIZE_ERROR;
+    if (rightEndpointsDataSize != I * es) return FAST_FSS_INVALID_BOUNDARY_DATA_SIZE_ERROR;
+    if (cache == nullptr || cacheDataSize != spline_cache_size(n, I, degree, es, N))
+        return FAST_FSS_INVALID_CACHE_DATA_SIZE_ERROR;
+    if (N == 0) return FAST_FSS_SUCCESS;
+    if (!aligned(key, 8) || !aligned(seed, 8)) return FAST_FSS_RUNTIME_ERROR;
+    hipStream_t st = resolve_stream(cudaStreamPtr);
+    if (degree <= 7)
+    {
+        FSS_DISPATCH(es, spline_eval_kernel<T><<<grid_for(N), kBlock, 0, st>>>(
+                             (T *)sharedOut, (const T *)maskedX, (const uint8_t *)key, (const T *)sharedE,
+                             (const T *)sharedBeta, (const uint8_t *)seed, partyId, (const T *)leftEndpoints,
+                             (const T *)rightEndpoints, I, (int)degree, (int)n, (int)m, N));
+    }
+    else
+    {
+        FSS_DISPATCH(es, spline_eval_wide_kernel<T><<<grid_for(N), kBlock, 0, st>>>(
+                             (T *)sharedOut, (const T *)maskedX, (const uint8_t *)key, (const T *)sharedE,
+                             (const T *)sharedBeta, (const uint8_t *)seed, partyId, (const T *)leftEndpoints,
+                             (const T *)rightEndpoints, I, (int)degree, (int)n, (int)m, N, (uint8_t *)cache));
+    }
+    return launch_status();
+}

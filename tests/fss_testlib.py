@@ -421,3 +421,56 @@ def ottt_eval(be, x, key, party, lut, n, es):
 def signed_nbit(v: int, bits: int) -> int:
     v &= mask(bits)
     return v - (1 << bits) if (bits and v >> (bits - 1)) else v
+
+
+class DeviceBackend:
+    """Runs the same argument lists through the device library's C ABI
+    (FastFss_cuda_*) on HBM buffers allocated by torch on cuda:0."""
+
+    name = "device"
+
+    def __init__(self):
+        import torch
+
+        from fastfunctionsecretsharing_amd import _lib
+
+        self.torch = torch
+        self.lib = _lib.load()
+        self.dev = torch.device("cuda:0")
+        self._stream = ctypes.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
+
+    def put(self, a):
+        if a is None:
+            return None
+        if hasattr(a, "data_ptr"):
+            return a.contiguous()
+        a = np.ascontiguousarray(a)
+        t = self.torch.from_numpy(a.view(np.uint8).copy()).to(self.dev)
+        return t.view(_TORCH_DT(self.torch)[a.dtype.itemsize]) if a.dtype != np.uint8 else t
+
+    def empty(self, nbytes, dtype=np.uint8):
+        it = np.dtype(dtype).itemsize
+        return self.torch.zeros(nbytes // it, dtype=_TORCH_DT(self.torch)[it], device=self.dev)
+
+    def get(self, t):
+        self.torch.cuda.synchronize()
+        it = t.element_size()
+        return t.view(self.torch.uint8).cpu().numpy().view(DT[it])
+
+    def call(self, name, *args):
+        if name.startswith(("dcfGet", "dpfGet", "grottoGet", "dcfMICGet", "dcfSplineGet", "otttGet")):
+            fn = getattr(self.lib, "FastFss_" + name)
+            return fn(*[ptr(a) if (a is None or hasattr(a, "data_ptr")) else a for a in args])
+        fn = getattr(self.lib, "FastFss_cuda_" + name)
+        conv = [ptr(a) if (a is None or hasattr(a, "data_ptr")) else a for a in args]
+        rc = fn(*conv, ctypes.addressof(self._stream))
+        return rc
+
+
+def _TORCH_DT(torch):
+    # signed torch dtypes carry the same bit patterns and support every op needed
+    return {1: torch.uint8, 2: torch.int16, 4: torch.int32, 8: torch.int64}
+
+
+def to_np(be, a):
+    return be.get(a) if hasattr(a, "data_ptr") else a

@@ -1,0 +1,129 @@
+"""DCF parity on the GPU: the device library against the oracle, bit-exact.
+
+Small and medium sizes compare key bytes and both parties' shares with the
+oracle element for element.  The 2^24-element n=32 configuration compares a
+random sample of elements with the oracle and checks the size-independent
+reconstruction property share0 + share1 == [x < alpha] on every element.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from fss_testlib import (DT, DeviceBackend, OracleBackend, dcf_eval, dcf_keygen, mask, ocall, rand_elems,
+                         rand_seeds)
+
+pytestmark = pytest.mark.gpu
+
+OR = OracleBackend()
+
+
+@pytest.fixture(scope="module")
+def dev():
+    return DeviceBackend()
+
+
+GRID = {1: [1, 7, 8], 2: [1, 15, 16], 4: [1, 31, 32], 8: [1, 63, 64]}
+
+
+def _case(rng, N, n, m, g, es):
+    alpha = rand_elems(rng, N, n, es)
+    x = rand_elems(rng, N, n, es)
+    beta = rand_elems(rng, N * g, m, es)
+    masked = ((x.astype(np.uint64) + alpha) & np.uint64(mask(n))).astype(DT[es])
+    return alpha, beta, masked, rand_seeds(rng, N), rand_seeds(rng, N)
+
+
+def _check(dev, alpha, beta, masked, s0, s1, n, m, g, es):
+    key_o = dcf_keygen(OR, alpha, beta, s0, s1, n, m, g, es)
+    key_d = dcf_keygen(dev, alpha, beta, s0, s1, n, m, g, es)
+    assert np.array_equal(dev.get(key_d), key_o), "keyGen bytes differ"
+    for party, seed in ((0, s0), (1, s1)):
+        y_o = dcf_eval(OR, masked, key_o, seed, party, n, m, g, es)
+        y_d = dev.get(dcf_eval(dev, masked, key_d, seed, party, n, m, g, es))
+        assert np.array_equal(y_d, y_o), f"party {party} shares differ"
+
+
+@pytest.mark.parametrize("es", [1, 2, 4, 8])
+def test_dcf_reference_grid(dev, es):
+    rng = np.random.default_rng(1000 + es)
+    for n, m in itertools.product(GRID[es], GRID[es]):
+        for g, N in [(1, 1023), (1, 1), (2, 8), (3, 33), (4, 2047)]:
+            _check(dev, *_case(rng, N, n, m, g, es), n, m, g, es)
+
+
+@pytest.mark.parametrize("n,m,g,es", [(24, 24, 6, 4), (32, 32, 5, 4), (16, 16, 9, 2), (24, 32, 48, 4),
+                                      (64, 64, 3, 8)])
+def test_dcf_aes_conversion_groups(dev, n, m, g, es):
+    """groupSize*bytes > 16 takes the reference's AES refill conversion."""
+    rng = np.random.default_rng(n * g)
+    _check(dev, *_case(rng, 300, n, m, g, es), n, m, g, es)
+
+
+def test_dcf_beta_null(dev):
+    rng = np.random.default_rng(9)
+    alpha, _, masked, s0, s1 = _case(rng, 500, 16, 16, 1, 2)
+    _check(dev, alpha, None, masked, s0, s1, 16, 16, 1, 2)
+
+
+def test_dcf_empty_and_errors(dev):
+    import torch
+
+    z = torch.zeros(4096, dtype=torch.uint8, device="cuda:0")
+    zn = np.zeros(4096, dtype=np.uint8)
+    ks = 16 * 8 + 8 + 3
+    cases = [
+        (z, 1, z, 1, z, ks, z, 16, 0, 8, 8, 1, 3, 1, None, 0),
+        (z, 1, z, 1, z, ks, z, 16, 0, 9, 8, 1, 1, 1, None, 0),
+        (z, 1, z, 1, z, ks, z, 16, 2, 8, 8, 1, 1, 1, None, 0),
+        (z, 1, z, 1, z, ks, z, 16, 0, 8, 8, 0, 1, 1, None, 0),
+        (z, 1, z, 1, z, ks + 1, z, 16, 0, 8, 8, 1, 1, 1, None, 0),
+        (z, 2, z, 1, z, ks, z, 16, 0, 8, 8, 1, 1, 1, None, 0),
+        (z, 1, z, 2, z, ks, z, 16, 0, 8, 8, 1, 1, 1, None, 0),
+        (z, 1, z, 1, z, ks, z, 15, 0, 8, 8, 1, 1, 1, None, 0),
+        (z, 1, z, 1, z, ks, z, 16, 0, 8, 8, 1, 1, 1, z, 7),
+    ]
+    for c in cases:
+        o = tuple(zn if (hasattr(a, "data_ptr")) else a for a in c)
+        assert dev.call("dcfEval", *c) == ocall("dcfEval", *o) != 0
+    # zero elements: success, nothing launched
+    assert dev.call("dcfEval", None, 0, None, 0, None, 0, None, 0, 0, 8, 8, 1, 1, 0, None, 0) == 0
+
+
+def _sub_key(key, idx, n, g, es, N):
+    """Key bytes of the elements `idx` as a standalone len(idx)-element key."""
+    sc = key[:N * 16 * n].reshape(N, 16 * n)[idx]
+    tl = key[N * 16 * n:].reshape(N, es * (n * g + 2 + g))[idx]
+    return np.concatenate([sc.reshape(-1), tl.reshape(-1)])
+
+
+def test_dcf_2pow24_n32(dev):
+    """Plain DCF, bitWidthIn = bitWidthOut = 32, 2^24 elements, one key each."""
+    import torch
+
+    N, n, m, g, es = 1 << 24, 32, 32, 1, 4
+    gen = torch.Generator(device="cuda:0").manual_seed(2024)
+    alpha = torch.randint(0, 2**32, (N,), dtype=torch.int64, device="cuda:0", generator=gen)
+    x = torch.randint(0, 2**32, (N,), dtype=torch.int64, device="cuda:0", generator=gen)
+    masked = (x + alpha) & 0xFFFFFFFF
+    seeds = torch.randint(0, 256, (2, 16 * N), dtype=torch.uint8, device="cuda:0", generator=gen)
+    a32 = alpha.to(torch.int32)  # bit patterns of the uint32 elements
+    m32 = masked.to(torch.int32)
+    key = dcf_keygen(dev, a32, None, seeds[0], seeds[1], n, m, g, es)
+    y0 = dcf_eval(dev, m32, key, seeds[0], 0, n, m, g, es)
+    y1 = dcf_eval(dev, m32, key, seeds[1], 1, n, m, g, es)
+    rec = (y0.to(torch.int64) + y1.to(torch.int64)) & 0xFFFFFFFF
+    want = (masked < alpha).to(torch.int64)
+    assert torch.equal(rec, want)
+    # bit-exact against the oracle on a sample of 4096 elements
+    idx = np.sort(np.random.default_rng(5).choice(N, 4096, replace=False))
+    key_h = dev.get(key)
+    sub = _sub_key(key_h, idx, n, g, es, N)
+    a_h = dev.get(a32)[idx]
+    s0_h = dev.get(seeds[0]).reshape(N, 16)[idx].reshape(-1)
+    s1_h = dev.get(seeds[1]).reshape(N, 16)[idx].reshape(-1)
+    key_o = dcf_keygen(OR, a_h, None, s0_h, s1_h, n, m, g, es)
+    assert np.array_equal(key_o, sub)
+    m_h = dev.get(m32)[idx]
+    assert np.array_equal(dcf_eval(OR, m_h, key_o, s0_h, 0, n, m, g, es), dev.get(y0)[idx])
+    assert np.array_equal(dcf_eval(OR, m_h, key_o, s1_h, 1, n, m, g, es), dev.get(y1)[idx])
