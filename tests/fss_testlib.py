@@ -394,6 +394,7 @@ def spline_eval(be, x, key, e, beta, seed, party, left, right, degree, n, m, es)
     cache = be.empty(max(cs, 1))
     out = be.empty(N * es, DT[es])
     xx, sd, lf, rt = be.put(x), be.put(seed), be.put(left), be.put(right)
+    e, beta = be.put(e), be.put(beta)
     run(be, "dcfSplineEval", out, nbytes(out), xx, nbytes(xx), key, nbytes(key), e, nbytes(e), beta, nbytes(beta),
         sd, nbytes(sd), party, lf, nbytes(lf), rt, nbytes(rt), I, degree, n, m, es, N, cache, cs)
     return out
