@@ -43,6 +43,10 @@ def parse():
     ap.add_argument("--log2n", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-sample-log2", type=int, default=22)
+    ap.add_argument("--no-ragged", action="store_true", help="skip the ragged-batch leg")
+    ap.add_argument("--ragged-groups", type=int, default=1024)
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="after timing, run one device-to-device copy of this many MiB (counter calibration)")
     return ap.parse_args()
 
 
@@ -79,6 +83,68 @@ def cpu_baseline(log2n: int):
         "kind": "port",
         "sample": f"DCF n=32 groupSize 1, 2^{log2n} elements, party 0, median of 3 runs, "
                   f"AES-NI={'yes' if lib.oracle_aes_uses_aesni() else 'no'}",
+    }
+
+
+def ragged_leg(groups_n: int, reps: int = 5):
+    """Mixed DCF/DPF groups (1024 groups, log-uniform 2^6..2^18 elements, widths
+    {8,16,24,32,64}, DCF:DPF 70:30): 1024 per-call launches on one stream vs one
+    persistent batched launch replayed from a hipGraph.  Party 0, keys resident."""
+    import torch
+
+    from fastfunctionsecretsharing_amd import _lib, workloads as W
+
+    lib = _lib.load()
+    groups = W.ragged_batch(groups=groups_n, seed=11)
+    total = sum(g.N for g in groups)
+    stream = torch.cuda.Stream()
+    holder = ctypes.c_void_p(stream.cuda_stream)
+    sp = ctypes.addressof(holder)
+    ref = W.alloc_outputs(groups)
+    outs = W.alloc_outputs(groups)
+    with torch.cuda.stream(stream):
+        W.eval_plain(groups, 0, ref, sp)  # warm-up and reference
+    stream.synchronize()
+    desc = W.batch_descriptors(groups, 0, outs)
+    plan = ctypes.c_void_p()
+    assert lib.FastFss_hip_batchPlanCreate(ctypes.byref(plan), desc, len(groups)) == 0
+    assert lib.FastFss_hip_batchPlanCapture(plan, sp) == 0
+    assert lib.FastFss_hip_batchPlanReplay(plan, sp) == 0
+    stream.synchronize()
+    same = all(torch.equal(a, b) for a, b in zip(outs, ref))
+
+    def timed(fn):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        stream.synchronize()
+        t0 = time.perf_counter()
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        stream.synchronize()
+        return (time.perf_counter() - t0) / reps, e0.elapsed_time(e1) / reps
+
+    with torch.cuda.stream(stream):
+        plain_s, plain_dev_ms = timed(lambda: W.eval_plain(groups, 0, ref, sp))
+        batch_s, batch_dev_ms = timed(lambda: lib.FastFss_hip_batchPlanReplay(plan, sp))
+    lib.FastFss_hip_batchPlanDestroy(plan)
+    widths = {}
+    for g in groups:
+        widths[g.n] = widths.get(g.n, 0) + 1
+    return {
+        "workload": "%d groups, sizes log-uniform 2^6..2^18, bitWidthIn=bitWidthOut in {8,16,24,32,64}, "
+                    "DCF:DPF 70:30, party 0" % groups_n,
+        "elements": total,
+        "plain_evals_per_s": total / plain_s,
+        "batched_evals_per_s": total / batch_s,
+        "speedup": plain_s / batch_s,
+        "plain_ms": plain_s * 1e3,
+        "batched_ms": batch_s * 1e3,
+        "batched_device_ms": batch_dev_ms,
+        "plain_device_ms": plain_dev_ms,
+        "batched_equals_plain": same,
+        "groups_per_width": widths,
+        "tiles": int(lib.FastFss_hip_batchPlanTileCount(plan)) if False else None,
     }
 
 
@@ -171,13 +237,24 @@ def main():
             "unit": "TOP/s",
             "frac": achieved_tops / VALU_PEAK_TOPS,
             "traffic": None,
-            "kernel": "dcf_eval_g1_kernel<uint32_t>",
+            "kernel": "dcf_eval_burst_kernel<uint32_t, 8, 1>",
             "kernel_ms": kernel_ms,
             "ops_per_eval": n * OPS_PER_LEVEL,
             "hbm_achieved_gbs": hbm_gbs,
             "hbm_frac": hbm_gbs / HBM_PEAK_GBS,
         },
     }
+    if not a.no_ragged:
+        del key, out
+        torch.cuda.empty_cache()
+        res["ragged"] = ragged_leg(a.ragged_groups)
+    if a.calib_copy_mib:
+        src = torch.empty(a.calib_copy_mib << 20, dtype=torch.uint8, device=dev).fill_(1)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        res["calib_copy_bytes"] = a.calib_copy_mib << 20
     if not a.no_cpu:
         res["cpu_baseline"] = cpu_baseline(a.cpu_sample_log2)
     print(json.dumps(res))

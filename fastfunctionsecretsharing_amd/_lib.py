@@ -67,10 +67,47 @@ SIGNATURES = {
     "FastFss_dcfSplineGetKeyDataSize": ("pssssss", "int"),
     "FastFss_dcfSplineGetZippedKeyDataSize": ("pssssss", "int"),
     "FastFss_otttGetKeyDataSize": ("pss", "int"),
+    # MI355X-only entry points (include/FastFss/hip/*.h)
+    "FastFss_hip_batchPlanCreate": ("pps", "int"),
+    "FastFss_hip_batchPlanLaunch": ("pp", "int"),
+    "FastFss_hip_batchPlanCapture": ("pp", "int"),
+    "FastFss_hip_batchPlanReplay": ("pp", "int"),
+    "FastFss_hip_batchPlanTileCount": ("p", "size"),
+    "FastFss_hip_batchPlanDestroy": ("p", None),
+    "FastFss_hip_batchEval": ("psp", "int"),
+    "FastFss_hip_batchSetDefaultTile": ("isu", "int"),
+    "FastFss_hip_batchGetDefaultTile": ("is", "uint"),
+    "FastFss_hip_batchSetBlocksPerCu": ("i", "int"),
+    "FastFss_hip_setEvalVariant": ("i", "int"),
+    "FastFss_hip_getEvalVariantCount": ("", "int"),
+    "FastFss_hip_setEvalVariantForElementSize": ("si", "int"),
 }
 
-_KIND = {"p": ctypes.c_void_p, "s": ctypes.c_size_t, "i": ctypes.c_int, "b": ctypes.c_bool}
-_RES = {"int": ctypes.c_int, "ptr": ctypes.c_void_p, None: None}
+
+class BatchGroup(ctypes.Structure):
+    """ctypes mirror of FastFssBatchGroup (include/FastFss/hip/batch.h)."""
+
+    _fields_ = [
+        ("op", ctypes.c_int),
+        ("partyId", ctypes.c_int),
+        ("bitWidthIn", ctypes.c_size_t),
+        ("bitWidthOut", ctypes.c_size_t),
+        ("elementSize", ctypes.c_size_t),
+        ("elementNum", ctypes.c_size_t),
+        ("sharedOut", ctypes.c_void_p),
+        ("maskedX", ctypes.c_void_p),
+        ("key", ctypes.c_void_p),
+        ("seed", ctypes.c_void_p),
+        ("keyDataSize", ctypes.c_size_t),
+        ("tileElems", ctypes.c_uint32),
+    ]
+
+
+BATCH_DCF_EVAL = 0
+BATCH_DPF_EVAL = 1
+
+_KIND = {"p": ctypes.c_void_p, "s": ctypes.c_size_t, "i": ctypes.c_int, "b": ctypes.c_bool, "u": ctypes.c_uint32}
+_RES = {"int": ctypes.c_int, "ptr": ctypes.c_void_p, "size": ctypes.c_size_t, "uint": ctypes.c_uint32, None: None}
 
 _lib = None
 

@@ -52,6 +52,45 @@ __global__ __launch_bounds__(kBlock) void dcf_eval_g1_kernel(T *out, const T *x,
     }
 }
 
+// E elements per lane: lane `gid` takes elements gid + k*S (k < E, S = total
+// lanes) per step of a grid-stride loop over E*S; surplus slots redo the
+// first element and are not stored.
+template <typename T, int CH, int E>
+__global__ __launch_bounds__(kBlock) void dcf_eval_burst_kernel(T *out, const T *x, const uint8_t *key,
+                                                                const uint8_t *seed, int party, int n, int outBytes,
+                                                                size_t N)
+{
+    __shared__ uint32_t lds[kLdsTableWords];
+    aes_fill_lds(lds);
+    __syncthreads();
+    const AesCtx c     = aes_ctx(lds);
+    const size_t sw    = 16 * (size_t)n;
+    const T     *tails = (const T *)(key + N * sw);
+    const size_t ts    = (size_t)n + 3;
+    const size_t S     = (size_t)gridDim.x * blockDim.x;
+    for (size_t e0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e0 < N; e0 += S * E)
+    {
+        size_t         idx[E];
+        const uint8_t *sc[E];
+        const T       *tl[E];
+        T              xv[E], o[E];
+        uint32_t       s[E][4];
+#pragma unroll
+        for (int k = 0; k < E; k++)
+        {
+            idx[k] = (e0 + k * S < N) ? e0 + k * S : e0;
+            sc[k]  = key + idx[k] * sw;
+            tl[k]  = tails + idx[k] * ts;
+            xv[k]  = x[idx[k]];
+            load16(s[k], seed + 16 * idx[k]);
+        }
+        dcf_eval_g1_burst<T, CH, E>(c, o, sc, tl, xv, s, party, n, outBytes);
+#pragma unroll
+        for (int k = 0; k < E; k++)
+            if (e0 + k * S < N) out[idx[k]] = o[k];
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void dcf_eval_group_kernel(T *out, const T *x, const uint8_t *key,
                                                                 const uint8_t *seed, int party, int n, int m, int g,
@@ -81,9 +120,63 @@ __global__ __launch_bounds__(kBlock) void dcf_eval_group_kernel(T *out, const T 
     }
 }
 
+// Groupsize-1 evaluation variants: 0 = per-level loads, 1..4 = bursts of
+// CH levels with E elements per lane (see dcf_eval_g1_burst).
+struct EvalVariant
+{
+    int ch, e;
+};
+constexpr EvalVariant kEvalVariants[] = {{1, 1}, {4, 1}, {8, 1}, {4, 2}, {2, 2}};
+constexpr int         kNumEvalVariants = sizeof(kEvalVariants) / sizeof(kEvalVariants[0]);
+
+static int g_variant_override = -1;           // FastFss_hip_setEvalVariant
+static int g_variant_table[9] = {-1, 2, 2, -1, 2, -1, -1, -1, 2}; // per element size, measured on MI355X
+
+int dcf_eval_variant(size_t es, size_t n, size_t N)
+{
+    (void)n, (void)N;
+    if (g_variant_override >= 0) return g_variant_override;
+    int v = (es < 9) ? g_variant_table[es] : -1;
+    return v < 0 ? 2 : v;
+}
+
+template <typename T>
+void launch_dcf_eval_g1(int v, int grid, hipStream_t st, T *out, const T *x, const uint8_t *key, const uint8_t *seed,
+                        int party, int n, int outBytes, size_t N)
+{
+    switch (v)
+    {
+        case 0: dcf_eval_g1_kernel<T><<<grid, kBlock, 0, st>>>(out, x, key, seed, party, n, outBytes, N); break;
+        case 2: dcf_eval_burst_kernel<T, 8, 1><<<grid, kBlock, 0, st>>>(out, x, key, seed, party, n, outBytes, N); break;
+        case 3: dcf_eval_burst_kernel<T, 4, 2><<<grid, kBlock, 0, st>>>(out, x, key, seed, party, n, outBytes, N); break;
+        case 4: dcf_eval_burst_kernel<T, 2, 2><<<grid, kBlock, 0, st>>>(out, x, key, seed, party, n, outBytes, N); break;
+        default: dcf_eval_burst_kernel<T, 4, 1><<<grid, kBlock, 0, st>>>(out, x, key, seed, party, n, outBytes, N); break;
+    }
+}
+
 } // namespace fss
 
 using namespace fss;
+
+extern "C" FAST_FSS_API int FastFss_hip_setEvalVariant(int variant)
+{
+    if (variant < -1 || variant >= kNumEvalVariants) return FAST_FSS_RUNTIME_ERROR;
+    g_variant_override = variant;
+    return FAST_FSS_SUCCESS;
+}
+
+extern "C" FAST_FSS_API int FastFss_hip_getEvalVariantCount(void)
+{
+    return kNumEvalVariants;
+}
+
+extern "C" FAST_FSS_API int FastFss_hip_setEvalVariantForElementSize(size_t elementSize, int variant)
+{
+    if (elementSize >= 9 || !elem_size_ok(elementSize) || variant < -1 || variant >= kNumEvalVariants)
+        return FAST_FSS_RUNTIME_ERROR;
+    g_variant_table[elementSize] = variant;
+    return FAST_FSS_SUCCESS;
+}
 
 extern "C" FAST_FSS_API int FastFss_cuda_dcfKeyGen(void *key, size_t keyDataSize, const void *alpha,
                                                    size_t alphaDataSize, const void *beta, size_t betaDataSize,
@@ -135,9 +228,10 @@ extern "C" FAST_FSS_API int FastFss_cuda_dcfEval(void *sharedOut, size_t sharedO
     hipStream_t st = resolve_stream(cudaStreamPtr);
     if (g == 1)
     {
-        FSS_DISPATCH(es, dcf_eval_g1_kernel<T><<<grid_for(N), kBlock, 0, st>>>(
-                             (T *)sharedOut, (const T *)maskedX, (const uint8_t *)key, (const uint8_t *)seed, partyId,
-                             (int)n, (int)((m + 7) / 8), N));
+        const int v = dcf_eval_variant(es, n, N);
+        FSS_DISPATCH(es, launch_dcf_eval_g1<T>(v, grid_for(N), st, (T *)sharedOut, (const T *)maskedX,
+                                               (const uint8_t *)key, (const uint8_t *)seed, partyId, (int)n,
+                                               (int)((m + 7) / 8), N));
     }
     else
     {

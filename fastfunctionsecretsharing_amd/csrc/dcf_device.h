@@ -67,6 +67,88 @@ __device__ __forceinline__ T dcf_eval_g1(const AesCtx &c, const uint8_t *sCW, co
     return (T)(acc + neg_if<T>(term, party));
 }
 
+// Groupsize-1 shares of E elements per lane with the correction words read in
+// bursts of CH levels.  A lane's sCW run is 16n contiguous bytes and its vCW
+// run n contiguous group elements; fetching CH levels back to back consumes a
+// whole 16*CH-byte stretch of a cache line while it is resident, instead of
+// touching each line once per level.  E > 1 interleaves independent elements
+// for instruction-level parallelism.  Results equal dcf_eval_g1 per element.
+template <typename T, int CH, int E>
+__device__ __forceinline__ void dcf_eval_g1_burst(const AesCtx &c, T out[E], const uint8_t *const sCW[E],
+                                                  const T *const tail[E], const T xin[E], uint32_t s[E][4], int party,
+                                                  int n, int outBytes)
+{
+    uint32_t t[E];
+    T        acc[E], x[E], tl[E], tr[E];
+#pragma unroll
+    for (int e = 0; e < E; e++)
+    {
+        s[e][0] &= ~1u;
+        t[e]   = (uint32_t)party;
+        x[e]   = mod_bits<T>(xin[e], n);
+        acc[e] = 0;
+        tl[e]  = tail[e][n];
+        tr[e]  = tail[e][n + 1];
+    }
+    for (int i0 = 0; i0 < n; i0 += CH)
+    {
+        uint32_t cw[E][CH][4];
+        T        vcw[E][CH];
+#pragma unroll
+        for (int e = 0; e < E; e++)
+        {
+#pragma unroll
+            for (int k = 0; k < CH; k++)
+            {
+                if (i0 + k < n)
+                {
+                    load16(cw[e][k], sCW[e] + 16 * (i0 + k));
+                    vcw[e][k] = tail[e][i0 + k];
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < CH; k++)
+        {
+            const int i = i0 + k;
+            if (i < n)
+            {
+                uint32_t blk[E][2][4];
+                uint32_t xi[E];
+#pragma unroll
+                for (int e = 0; e < E; e++)
+                {
+                    xi[e]        = (uint32_t)(x[e] >> (n - 1 - i)) & 1u;
+                    blk[e][0][0] = 4u * xi[e], blk[e][0][1] = 0u, blk[e][0][2] = 4u * xi[e] + 1u, blk[e][0][3] = 0u;
+                    blk[e][1][0] = 4u * xi[e] + 2u, blk[e][1][1] = 0u, blk[e][1][2] = 4u * xi[e] + 3u,
+                    blk[e][1][3] = 0u;
+                }
+                aes_encrypt_multi<E, 2>(c, s, blk);
+#pragma unroll
+                for (int e = 0; e < E; e++)
+                {
+                    const uint32_t m  = 0u - t[e];
+                    uint32_t       tn = blk[e][0][0] & 1u;
+                    s[e][0]           = (blk[e][0][0] & ~1u) ^ (cw[e][k][0] & m);
+                    s[e][1]           = blk[e][0][1] ^ (cw[e][k][1] & m);
+                    s[e][2]           = blk[e][0][2] ^ (cw[e][k][2] & m);
+                    s[e][3]           = blk[e][0][3] ^ (cw[e][k][3] & m);
+                    tn ^= (uint32_t)(((xi[e] ? tr[e] : tl[e]) >> i) & 1) & t[e];
+                    T term = (T)((T)low_bytes(blk[e][1], outBytes) + (t[e] ? vcw[e][k] : (T)0));
+                    acc[e] = (T)(acc[e] + neg_if<T>(term, party));
+                    t[e]   = tn;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < E; e++)
+    {
+        T term = (T)((T)low_bytes(s[e], outBytes) + (t[e] ? tail[e][n + 2] : (T)0));
+        out[e] = (T)(acc[e] + neg_if<T>(term, party));
+    }
+}
+
 // Share for group elements [j0, j0 + cnt) of a groupSize-g DCF; cnt <= R.
 // Used for g > 1 (R covers the group) and by the spline evaluator, which
 // needs only the coefficient ranges of one or two intervals.

@@ -233,6 +233,43 @@ __device__ __forceinline__ void aes_encrypt(const AesCtx &c, const uint32_t key[
     for (int b = 0; b < NB; b++) aes_last(c, st[b], k);
 }
 
+// E independent keys, NB blocks each, rounds interleaved across keys so the
+// lookups of all E*NB states are in flight together.
+template <int E, int NB>
+__device__ __forceinline__ void aes_encrypt_multi(const AesCtx &c, const uint32_t key[E][4], uint32_t st[E][NB][4])
+{
+    uint32_t k[E][4];
+#pragma unroll
+    for (int e = 0; e < E; e++)
+    {
+#pragma unroll
+        for (int i = 0; i < 4; i++) k[e][i] = key[e][i];
+#pragma unroll
+        for (int b = 0; b < NB; b++)
+            st[e][b][0] ^= k[e][0], st[e][b][1] ^= k[e][1], st[e][b][2] ^= k[e][2], st[e][b][3] ^= k[e][3];
+    }
+    const uint32_t rcon[10] = {0x01, 0x02, 0x04, 0x08, 0x10, 0x20, 0x40, 0x80, 0x1b, 0x36};
+#pragma unroll
+    for (int r = 0; r < 9; r++)
+    {
+#pragma unroll
+        for (int e = 0; e < E; e++)
+        {
+            aes_next_key(c, k[e], rcon[r]);
+            uint32_t h[4] = {rot16(k[e][0]), rot16(k[e][1]), rot16(k[e][2]), rot16(k[e][3])};
+#pragma unroll
+            for (int b = 0; b < NB; b++) aes_round(c, st[e][b], h);
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < E; e++)
+    {
+        aes_next_key(c, k[e], rcon[9]);
+#pragma unroll
+        for (int b = 0; b < NB; b++) aes_last(c, st[e][b], k[e]);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // PRG and value conversion
 // ---------------------------------------------------------------------------
