@@ -144,7 +144,6 @@ def ragged_leg(groups_n: int, reps: int = 5):
         "plain_device_ms": plain_dev_ms,
         "batched_equals_plain": same,
         "groups_per_width": widths,
-        "tiles": int(lib.FastFss_hip_batchPlanTileCount(plan)) if False else None,
     }
 
 
