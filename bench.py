@@ -45,6 +45,9 @@ def parse():
     ap.add_argument("--cpu-sample-log2", type=int, default=22)
     ap.add_argument("--no-ragged", action="store_true", help="skip the ragged-batch leg")
     ap.add_argument("--ragged-groups", type=int, default=1024)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serve-qps", type=float, default=400.0)
+    ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after timing, run one device-to-device copy of this many MiB (counter calibration)")
     return ap.parse_args()
@@ -147,6 +150,46 @@ def ragged_leg(groups_n: int, reps: int = 5):
     }
 
 
+def serving_leg(qps: float, requests: int):
+    """Poisson arrivals at `qps`; each request a DCF, MIC (16 intervals) or
+    spline (degree 2, 4 intervals) evaluation of 2^10..2^20 elements
+    (log-uniform), bitWidthIn in {16, 32, 64}; keys resident in HBM.  The same
+    trace runs through the continuous-batching server and through the serial
+    per-call form (one request at a time, persistent per-call launch shape)."""
+    import numpy as np
+    import torch
+
+    from fastfunctionsecretsharing_amd import workloads as W
+
+    templates = W.serving_templates(count=48, seed=5)
+    rng = np.random.default_rng(17)
+    picks = [int(p) for p in rng.integers(0, len(templates), size=requests)]
+    arrivals = np.cumsum(rng.exponential(1.0 / qps, size=requests))
+    # warm the kernels once
+    W.run_trace(templates, picks[:6], np.zeros(6), 1 << 35, 8, 4, 1)
+    res = {"workload": "Poisson %.0f req/s, %d requests; DCF/MIC/spline 1:1:1, 2^10..2^20 elements log-uniform, "
+                       "bitWidthIn in {16,32,64}; keys resident" % (qps, requests)}
+    outs = {}
+    for name, cfg in (("batched", (32 << 30, 16, 8, 1)), ("plain", (32 << 30, 1, 1, 0))):
+        o, lats, wall, st = W.run_trace(templates, picks, arrivals, *cfg)
+        outs[name] = o
+        elems = sum(templates[p].N for p in picks)
+        lat_ms = np.array(lats) * 1e3
+        res[name] = {
+            "evals_per_s": elems / wall,
+            "p50_ms": float(np.percentile(lat_ms, 50)),
+            "p99_ms": float(np.percentile(lat_ms, 99)),
+            "mean_ms": float(lat_ms.mean()),
+            "wall_s": wall,
+            "max_inflight": int(st.maxInflightSeen),
+            "config": {"hbm_budget_bytes": cfg[0], "max_inflight": cfg[1], "streams": cfg[2],
+                       "launch": "one workgroup per 1024 elements" if cfg[3] else "persistent, one workgroup per CU"},
+        }
+    res["outputs_equal"] = all(torch.equal(a, b) for a, b in zip(outs["batched"], outs["plain"]))
+    res["p99_speedup"] = res["plain"]["p99_ms"] / res["batched"]["p99_ms"]
+    return res
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -247,6 +290,9 @@ def main():
         del key, out
         torch.cuda.empty_cache()
         res["ragged"] = ragged_leg(a.ragged_groups)
+    if not a.no_serve:
+        torch.cuda.empty_cache()
+        res["serving"] = serving_leg(a.serve_qps, a.serve_requests)
     if a.calib_copy_mib:
         src = torch.empty(a.calib_copy_mib << 20, dtype=torch.uint8, device=dev).fill_(1)
         dst = torch.empty_like(src)

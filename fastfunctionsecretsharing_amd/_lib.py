@@ -81,7 +81,52 @@ SIGNATURES = {
     "FastFss_hip_setEvalVariant": ("i", "int"),
     "FastFss_hip_getEvalVariantCount": ("", "int"),
     "FastFss_hip_setEvalVariantForElementSize": ("si", "int"),
+    "FastFss_hip_serverCreate": ("psiii", "int"),
+    "FastFss_hip_serverSubmit": ("ppp", "int"),
+    "FastFss_hip_serverQuery": ("pq", "int"),
+    "FastFss_hip_serverWait": ("pqp", "int"),
+    "FastFss_hip_serverDrain": ("p", "int"),
+    "FastFss_hip_serverStats": ("pp", "int"),
+    "FastFss_hip_serverDestroy": ("p", None),
 }
+
+
+class ServeRequest(ctypes.Structure):
+    """ctypes mirror of FastFssServeRequest (include/FastFss/hip/serve.h)."""
+
+    _fields_ = [
+        ("op", ctypes.c_int),
+        ("partyId", ctypes.c_int),
+        ("bitWidthIn", ctypes.c_size_t),
+        ("bitWidthOut", ctypes.c_size_t),
+        ("elementSize", ctypes.c_size_t),
+        ("elementNum", ctypes.c_size_t),
+        ("sharedOut", ctypes.c_void_p),
+        ("maskedX", ctypes.c_void_p),
+        ("key", ctypes.c_void_p),
+        ("keyDataSize", ctypes.c_size_t),
+        ("seed", ctypes.c_void_p),
+        ("sharedZ", ctypes.c_void_p),
+        ("sharedBeta", ctypes.c_void_p),
+        ("leftEndpoints", ctypes.c_void_p),
+        ("rightEndpoints", ctypes.c_void_p),
+        ("intervalNum", ctypes.c_size_t),
+        ("degree", ctypes.c_size_t),
+    ]
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [
+        ("submitted", ctypes.c_uint64),
+        ("completed", ctypes.c_uint64),
+        ("failed", ctypes.c_uint64),
+        ("maxInflightSeen", ctypes.c_uint64),
+        ("elementsDone", ctypes.c_uint64),
+        ("busySeconds", ctypes.c_double),
+    ]
+
+
+SERVE_DCF, SERVE_MIC, SERVE_SPLINE = 0, 1, 2
 
 
 class BatchGroup(ctypes.Structure):
@@ -106,7 +151,8 @@ class BatchGroup(ctypes.Structure):
 BATCH_DCF_EVAL = 0
 BATCH_DPF_EVAL = 1
 
-_KIND = {"p": ctypes.c_void_p, "s": ctypes.c_size_t, "i": ctypes.c_int, "b": ctypes.c_bool, "u": ctypes.c_uint32}
+_KIND = {"p": ctypes.c_void_p, "s": ctypes.c_size_t, "i": ctypes.c_int, "b": ctypes.c_bool, "u": ctypes.c_uint32,
+         "q": ctypes.c_uint64}
 _RES = {"int": ctypes.c_int, "ptr": ctypes.c_void_p, "size": ctypes.c_size_t, "uint": ctypes.c_uint32, None: None}
 
 _lib = None

@@ -8,6 +8,7 @@
 #include "FastFss/cuda/dcf.h"
 #include "dcf_device.h"
 #include "fss_host.h"
+#include "fss_launch.h"
 
 namespace fss {
 
@@ -152,6 +153,15 @@ void launch_dcf_eval_g1(int v, int grid, hipStream_t st, T *out, const T *x, con
         case 4: dcf_eval_burst_kernel<T, 2, 2><<<grid, kBlock, 0, st>>>(out, x, key, seed, party, n, outBytes, N); break;
         default: dcf_eval_burst_kernel<T, 4, 1><<<grid, kBlock, 0, st>>>(out, x, key, seed, party, n, outBytes, N); break;
     }
+}
+
+int launch_dcf_eval_g1_rt(size_t es, int grid, hipStream_t st, void *out, const void *x, const void *key,
+                          const void *seed, int party, int n, int m, size_t N)
+{
+    const int v = dcf_eval_variant(es, (size_t)n, N);
+    FSS_DISPATCH(es, launch_dcf_eval_g1<T>(v, grid, st, (T *)out, (const T *)x, (const uint8_t *)key,
+                                           (const uint8_t *)seed, party, n, (m + 7) / 8, N));
+    return launch_status();
 }
 
 } // namespace fss

@@ -1,0 +1,25 @@
+// Internal launchers shared between translation units (not part of the C ABI).
+// `grid` = number of 1024-thread workgroups; the kernels loop grid-stride, so
+// any grid >= 1 is correct.  The public entry points pass the persistent
+// one-per-CU grid; the serving runtime passes one workgroup per 1024
+// elements so that concurrent requests' workgroups interleave on the CUs.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+namespace fss {
+
+int launch_dcf_eval_g1_rt(size_t es, int grid, hipStream_t st, void *out, const void *x, const void *key,
+                          const void *seed, int party, int n, int m, size_t N);
+
+int launch_mic_eval_rt(size_t es, int grid, hipStream_t st, void *out, const void *x, const void *key, const void *z,
+                       const void *seed, int party, const void *left, const void *right, size_t I, int n, int m,
+                       size_t N);
+
+// degree <= 7 (the path that needs no per-element workspace)
+int launch_spline_eval_rt(size_t es, int grid, hipStream_t st, void *out, const void *x, const void *key,
+                          const void *e, const void *beta, const void *seed, int party, const void *left,
+                          const void *right, size_t I, int degree, int n, int m, size_t N);
+
+} // namespace fss

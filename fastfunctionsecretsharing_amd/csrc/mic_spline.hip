@@ -11,6 +11,7 @@
 #include "FastFss/cuda/spline.h"
 #include "dcf_device.h"
 #include "fss_host.h"
+#include "fss_launch.h"
 
 namespace fss {
 
@@ -284,6 +285,39 @@ __global__ __launch_bounds__(kBlock) void spline_eval_wide_kernel(T *out, const 
     }
 }
 
+static size_t mic_blocks_per_elem(size_t N, size_t I)
+{
+    // split each element's intervals until the launch has a lane per CU slot
+    const size_t lanes = (size_t)device_cu_count() * kBlock;
+    size_t       bpe   = (lanes + N - 1) / N;
+    if (bpe < 1) bpe = 1;
+    if (bpe > I) bpe = I;
+    return bpe;
+}
+
+int launch_mic_eval_rt(size_t es, int grid, hipStream_t st, void *out, const void *x, const void *key, const void *z,
+                       const void *seed, int party, const void *left, const void *right, size_t I, int n, int m,
+                       size_t N)
+{
+    const size_t bpe = mic_blocks_per_elem(N, I);
+    FSS_DISPATCH(es, mic_eval_kernel<T><<<grid, kBlock, 0, st>>>((T *)out, (const T *)x, (const uint8_t *)key,
+                                                                 (const T *)z, (const uint8_t *)seed, party,
+                                                                 (const T *)left, (const T *)right, I, bpe, n,
+                                                                 (m + 7) / 8, N));
+    return launch_status();
+}
+
+int launch_spline_eval_rt(size_t es, int grid, hipStream_t st, void *out, const void *x, const void *key,
+                          const void *e, const void *beta, const void *seed, int party, const void *left,
+                          const void *right, size_t I, int degree, int n, int m, size_t N)
+{
+    if (degree > 7) return FAST_FSS_RUNTIME_ERROR;
+    FSS_DISPATCH(es, spline_eval_kernel<T><<<grid, kBlock, 0, st>>>(
+                         (T *)out, (const T *)x, (const uint8_t *)key, (const T *)e, (const T *)beta,
+                         (const uint8_t *)seed, party, (const T *)left, (const T *)right, I, degree, n, m, N));
+    return launch_status();
+}
+
 } // namespace fss
 
 using namespace fss;
@@ -343,11 +377,7 @@ extern "C" FAST_FSS_API int FastFss_cuda_dcfMICEval(void *sharedOut, size_t shar
     if (N == 0 || I == 0) return FAST_FSS_SUCCESS;
     if (!aligned(key, 8) || !aligned(seed, 8)) return FAST_FSS_RUNTIME_ERROR;
     hipStream_t st = resolve_stream(cudaStreamPtr);
-    // split each element's intervals until the launch has a lane per CU slot
-    const size_t lanes = (size_t)device_cu_count() * kBlock;
-    size_t       bpe   = (lanes + N - 1) / N;
-    if (bpe < 1) bpe = 1;
-    if (bpe > I) bpe = I;
+    const size_t bpe = mic_blocks_per_elem(N, I);
     FSS_DISPATCH(es, mic_eval_kernel<T><<<grid_for(N * bpe), kBlock, 0, st>>>(
                          (T *)sharedOut, (const T *)maskedX, (const uint8_t *)key, (const T *)sharedZ,
                          (const uint8_t *)seed, partyId, (const T *)leftEndpoints, (const T *)rightEndpoints, I, bpe,

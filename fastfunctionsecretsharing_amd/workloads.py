@@ -150,3 +150,199 @@ def reconstruct_ok(groups, outs0, outs1):
         if not torch.equal(s, want.to(torch.int64) & M):
             return False
     return True
+
+
+# ---------------------------------------------------------------------------
+# serving workload: DCF / MIC / spline requests with resident keys
+# ---------------------------------------------------------------------------
+
+MIC_INTERVALS = 16
+SPLINE_INTERVALS, SPLINE_DEGREE = 4, 2
+
+
+@dataclass
+class Template:
+    op: int  # _lib.SERVE_*
+    n: int
+    es: int
+    N: int
+    masked: object
+    key: object
+    key_size: int
+    seed0: object
+    z: object = None  # MIC z / spline e (party 0's share: the whole value)
+    beta: object = None
+    left: object = None
+    right: object = None
+    intervals: int = 0
+    degree: int = 0
+
+    def out_elems(self):
+        return self.N * (self.intervals if self.op == _lib.SERVE_MIC else 1)
+
+
+def _sorted_intervals(rng, I, n):
+    """Contiguous intervals covering [0, 2^n): left_0 = 0, right_{I-1} = 2^n - 1."""
+    M = (1 << n) - 1
+    cuts = sorted(set(int(v) for v in rng.integers(0, M, size=4 * I, dtype=np.uint64)))[: I - 1]
+    while len(cuts) < I - 1:
+        cuts.append(cuts[-1] + 1 if cuts else 0)
+    right = cuts + [M]
+    left = [0] + [c + 1 for c in cuts]
+    return left, right
+
+
+def make_template(op, n, N, rng, gen, dev, sp):
+    import torch
+
+    lib = _lib.load()
+    es = 2 if n <= 16 else (4 if n <= 32 else 8)
+    dt = torch_dtype(es)
+    alpha = _rand_bits(gen, N, n, dev).to(dt)
+    masked = _rand_bits(gen, N, n, dev).to(dt)
+    seeds = torch.randint(0, 256, (2, 16 * N), dtype=torch.uint8, device=dev, generator=gen)
+    ks = ctypes.c_size_t(0)
+
+    def ep(vals):
+        return torch.tensor([v if v < 2**63 else v - 2**64 for v in vals], dtype=torch.int64).to(dt).to(dev)
+
+    if op == _lib.SERVE_DCF:
+        assert lib.FastFss_dcfGetKeyDataSize(ctypes.addressof(ks), n, n, 1, es, N) == 0
+        key = torch.empty(ks.value, dtype=torch.uint8, device=dev)
+        rc = lib.FastFss_cuda_dcfKeyGen(key.data_ptr(), ks.value, alpha.data_ptr(), es * N, None, 0,
+                                        seeds[0].data_ptr(), 16 * N, seeds[1].data_ptr(), 16 * N, n, n, 1, es, N, sp)
+        assert rc == 0, rc
+        return Template(op, n, es, N, masked, key, ks.value, seeds[0])
+    I = MIC_INTERVALS if op == _lib.SERVE_MIC else SPLINE_INTERVALS
+    left, right = _sorted_intervals(rng, I, n)
+    lt, rt = ep(left), ep(right)
+    if op == _lib.SERVE_MIC:
+        assert lib.FastFss_dcfMICGetKeyDataSize(ctypes.addressof(ks), n, n, es, N) == 0
+        key = torch.empty(ks.value, dtype=torch.uint8, device=dev)
+        z = torch.empty(N * I, dtype=dt, device=dev)
+        rc = lib.FastFss_cuda_dcfMICKeyGen(key.data_ptr(), ks.value, z.data_ptr(), z.numel() * es, alpha.data_ptr(),
+                                           es * N, seeds[0].data_ptr(), 16 * N, seeds[1].data_ptr(), 16 * N,
+                                           lt.data_ptr(), es * I, rt.data_ptr(), es * I, n, n, es, N, sp)
+        assert rc == 0, rc
+        return Template(op, n, es, N, masked, key, ks.value, seeds[0], z=z, left=lt, right=rt, intervals=I)
+    d = SPLINE_DEGREE
+    g = I * (d + 1)
+    coeff = _rand_bits(gen, g, n, dev).to(dt)
+    assert lib.FastFss_dcfSplineGetKeyDataSize(ctypes.addressof(ks), d, I, n, n, es, N) == 0
+    key = torch.empty(ks.value, dtype=torch.uint8, device=dev)
+    e = torch.empty(N * g, dtype=dt, device=dev)
+    beta = torch.empty(N * g, dtype=dt, device=dev)
+    rc = lib.FastFss_cuda_dcfSplineKeyGen(key.data_ptr(), ks.value, e.data_ptr(), e.numel() * es, beta.data_ptr(),
+                                          beta.numel() * es, alpha.data_ptr(), es * N, seeds[0].data_ptr(), 16 * N,
+                                          seeds[1].data_ptr(), 16 * N, coeff.data_ptr(), g * es, d, lt.data_ptr(),
+                                          es * I, rt.data_ptr(), es * I, I, n, n, es, N, sp)
+    assert rc == 0, rc
+    return Template(op, n, es, N, masked, key, ks.value, seeds[0], z=e, beta=beta, left=lt, right=rt,
+                    intervals=I, degree=d)
+
+
+def serving_templates(count=48, seed=5, lo=10, hi=20, widths=(16, 32, 64), device="cuda:0"):
+    """`count` resident key sets, op in {DCF, MIC, spline} round robin, size
+    log-uniform in [2^lo, 2^hi], bitWidthIn = bitWidthOut drawn from `widths`."""
+    import torch
+
+    dev = torch.device(device)
+    rng = np.random.default_rng(seed)
+    gen = torch.Generator(device=dev).manual_seed(seed)
+    holder, sp = stream_ptr()
+    out = []
+    for i in range(count):
+        op = (_lib.SERVE_DCF, _lib.SERVE_MIC, _lib.SERVE_SPLINE)[i % 3]
+        N = int(round(math.exp(rng.uniform(math.log(2**lo), math.log(2**hi)))))
+        n = int(rng.choice(widths))
+        out.append(make_template(op, n, N, rng, gen, dev, sp))
+    torch.cuda.synchronize()
+    del holder
+    return out
+
+
+def serve_request(t, out):
+    r = _lib.ServeRequest()
+    r.op, r.partyId = t.op, 0
+    r.bitWidthIn = r.bitWidthOut = t.n
+    r.elementSize, r.elementNum = t.es, t.N
+    r.sharedOut = out.data_ptr()
+    r.maskedX = t.masked.data_ptr()
+    r.key, r.keyDataSize = t.key.data_ptr(), t.key_size
+    r.seed = t.seed0.data_ptr()
+    if t.op != _lib.SERVE_DCF:
+        r.sharedZ = t.z.data_ptr()
+        r.leftEndpoints, r.rightEndpoints = t.left.data_ptr(), t.right.data_ptr()
+        r.intervalNum = t.intervals
+    if t.op == _lib.SERVE_SPLINE:
+        r.sharedBeta, r.degree = t.beta.data_ptr(), t.degree
+    return r
+
+
+def eval_template_alone(t, out, sp):
+    """The per-call public entry point for one request (the reference-shaped launch)."""
+    import torch
+
+    lib = _lib.load()
+    es, N, n = t.es, t.N, t.n
+    if t.op == _lib.SERVE_DCF:
+        rc = lib.FastFss_cuda_dcfEval(out.data_ptr(), es * N, t.masked.data_ptr(), es * N, t.key.data_ptr(),
+                                      t.key_size, t.seed0.data_ptr(), 16 * N, 0, n, n, 1, es, N, None, 0, sp)
+    elif t.op == _lib.SERVE_MIC:
+        I = t.intervals
+        rc = lib.FastFss_cuda_dcfMICEval(out.data_ptr(), es * N * I, t.masked.data_ptr(), es * N, t.key.data_ptr(),
+                                         t.key_size, t.z.data_ptr(), es * N * I, t.seed0.data_ptr(), 16 * N, 0,
+                                         t.left.data_ptr(), es * I, t.right.data_ptr(), es * I, n, n, es, N, None, 0,
+                                         sp)
+    else:
+        I, d = t.intervals, t.degree
+        cs = ctypes.c_size_t(0)
+        assert lib.FastFss_dcfSplineGetCacheDataSize(ctypes.addressof(cs), d, I, n, n, es, N) == 0
+        cache = torch.empty(cs.value, dtype=torch.uint8, device=out.device)
+        g = I * (d + 1)
+        rc = lib.FastFss_cuda_dcfSplineEval(out.data_ptr(), es * N, t.masked.data_ptr(), es * N, t.key.data_ptr(),
+                                            t.key_size, t.z.data_ptr(), es * N * g, t.beta.data_ptr(), es * N * g,
+                                            t.seed0.data_ptr(), 16 * N, 0, t.left.data_ptr(), es * I,
+                                            t.right.data_ptr(), es * I, I, d, n, n, es, N, cache.data_ptr(),
+                                            cs.value, sp)
+        torch.cuda.synchronize()
+        del cache
+    if rc != 0:
+        raise RuntimeError(f"eval failed: {rc}")
+
+
+def run_trace(templates, picks, arrivals, budget, max_inflight, streams, launch_mode):
+    """Submit request i (template picks[i]) at time arrivals[i] seconds; return
+    (outputs, latencies, wall seconds, stats)."""
+    import time
+
+    import torch
+
+    lib = _lib.load()
+    outs = [torch.empty(templates[p].out_elems(), dtype=torch_dtype(templates[p].es), device=templates[p].key.device)
+            for p in picks]
+    reqs = [serve_request(templates[p], o) for p, o in zip(picks, outs)]
+    torch.cuda.synchronize()
+    srv = ctypes.c_void_p()
+    assert lib.FastFss_hip_serverCreate(ctypes.byref(srv), budget, max_inflight, streams, launch_mode) == 0
+    tickets = []
+    t0 = time.perf_counter()
+    for r, at in zip(reqs, arrivals):
+        delay = at - (time.perf_counter() - t0)
+        if delay > 0:
+            time.sleep(delay)
+        tk = ctypes.c_uint64()
+        rc = lib.FastFss_hip_serverSubmit(srv, ctypes.byref(r), ctypes.byref(tk))
+        assert rc == 0, rc
+        tickets.append(tk.value)
+    assert lib.FastFss_hip_serverDrain(srv) == 0
+    wall = time.perf_counter() - t0
+    lats = []
+    for tk in tickets:
+        lat = ctypes.c_double()
+        assert lib.FastFss_hip_serverWait(srv, tk, ctypes.byref(lat)) == 0
+        lats.append(lat.value)
+    st = _lib.ServeStats()
+    lib.FastFss_hip_serverStats(srv, ctypes.byref(st))
+    lib.FastFss_hip_serverDestroy(srv)
+    return outs, lats, wall, st

@@ -1,0 +1,97 @@
+/* Continuous-batching request server on MI355X (no counterpart in the
+ * reference, whose C API is one synchronous-launch call per evaluation).
+ *
+ * Requests (DCF, DCF-MIC or DCF-spline evaluations whose keys and inputs are
+ * already resident in device memory) are submitted from any host thread.  A
+ * scheduler thread admits queued requests in arrival order while
+ *   - the number of requests in flight is below `maxInflight`, and
+ *   - the bytes they touch (key + inputs + outputs) fit the HBM budget
+ *     (a request larger than the budget is admitted alone),
+ * launches each admitted request on one of `numStreams` HIP streams and
+ * retires it when its completion event fires.  Work of concurrently admitted
+ * requests overlaps on the device: with launchMode 1 every request uses one
+ * workgroup per 1024 elements, so the hardware interleaves the workgroups of
+ * all in-flight requests across the CUs.
+ *
+ * Each request's output is bit-identical to the per-call entry point with
+ * the same arguments (FastFss_cuda_dcfEval, FastFss_cuda_dcfMICEval,
+ * FastFss_cuda_dcfSplineEval).  Latency = completion observed by the
+ * scheduler minus submission.
+ */
+#ifndef FAST_FSS_HIP_SERVE_H
+#define FAST_FSS_HIP_SERVE_H
+
+#include <FastFss/api.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum FastFssServeOp
+{
+    FAST_FSS_SERVE_DCF_EVAL    = 0, /* groupSize 1 */
+    FAST_FSS_SERVE_MIC_EVAL    = 1,
+    FAST_FSS_SERVE_SPLINE_EVAL = 2, /* degree <= 7 */
+};
+
+typedef struct FastFssServeRequest
+{
+    int         op;
+    int         partyId;
+    size_t      bitWidthIn;
+    size_t      bitWidthOut;
+    size_t      elementSize;
+    size_t      elementNum;
+    void       *sharedOut;
+    const void *maskedX;
+    const void *key;
+    size_t      keyDataSize;
+    const void *seed;
+    const void *sharedZ;        /* MIC: z shares; spline: e shares */
+    const void *sharedBeta;     /* spline only */
+    const void *leftEndpoints;  /* MIC / spline */
+    const void *rightEndpoints; /* MIC / spline */
+    size_t      intervalNum;    /* MIC / spline */
+    size_t      degree;         /* spline */
+} FastFssServeRequest;
+
+typedef struct FastFssServeStats
+{
+    uint64_t submitted;
+    uint64_t completed;
+    uint64_t failed;
+    uint64_t maxInflightSeen;
+    uint64_t elementsDone;
+    double   busySeconds; /* wall time with at least one request in flight */
+} FastFssServeStats;
+
+/* launchMode: 0 = one persistent workgroup per CU (the per-call entry points'
+ * shape), 1 = one workgroup per 1024 elements (interleaves across requests). */
+FAST_FSS_API int FastFss_hip_serverCreate(void **server, size_t hbmBudgetBytes, int maxInflight, int numStreams,
+                                          int launchMode);
+
+/* Validate and enqueue; *ticket identifies the request.  Non-blocking. */
+FAST_FSS_API int FastFss_hip_serverSubmit(void *server, const FastFssServeRequest *req, uint64_t *ticket);
+
+/* 1 if done, 0 if pending or running, negative FastFssError if it failed. */
+FAST_FSS_API int FastFss_hip_serverQuery(void *server, uint64_t ticket);
+
+/* Block until the request is done; returns its status and, if latencySeconds
+ * is not NULL, its submit-to-completion latency. */
+FAST_FSS_API int FastFss_hip_serverWait(void *server, uint64_t ticket, double *latencySeconds);
+
+/* Block until every submitted request is done. */
+FAST_FSS_API int FastFss_hip_serverDrain(void *server);
+
+FAST_FSS_API int FastFss_hip_serverStats(void *server, FastFssServeStats *stats);
+
+/* Drain, stop the scheduler thread and release the server. */
+FAST_FSS_API void FastFss_hip_serverDestroy(void *server);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,71 @@
+"""Serving runtime: continuous admission under an HBM budget, overlapping
+requests on several streams; every request's output equals the same request
+evaluated alone through the per-call entry point."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def env():
+    import torch
+
+    from fastfunctionsecretsharing_amd import _lib, workloads as W
+
+    torch.cuda.set_device(0)
+    templates = W.serving_templates(count=12, seed=9, lo=6, hi=13)
+    return torch, _lib, W, templates
+
+
+def _alone(torch, W, templates, picks):
+    holder, sp = W.stream_ptr()
+    outs = []
+    for p in picks:
+        t = templates[p]
+        o = torch.empty(t.out_elems(), dtype=W.torch_dtype(t.es), device="cuda:0")
+        W.eval_template_alone(t, o, sp)
+        outs.append(o)
+    torch.cuda.synchronize()
+    del holder
+    return outs
+
+
+@pytest.mark.parametrize("budget,inflight,streams,mode", [(1 << 34, 16, 4, 1), (1 << 20, 4, 2, 1), (1 << 34, 1, 1, 0)])
+def test_mix_equals_alone(env, budget, inflight, streams, mode):
+    torch, _lib, W, templates = env
+    rng = np.random.default_rng(budget % 97 + inflight)
+    picks = [int(p) for p in rng.integers(0, len(templates), size=40)]
+    arrivals = np.cumsum(rng.exponential(0.0005, size=len(picks)))
+    arrivals[:10] = 0.0  # a burst: these are admitted together and overlap
+    outs, lats, wall, st = W.run_trace(templates, picks, arrivals, budget, inflight, streams, mode)
+    assert st.completed == len(picks) and st.failed == 0
+    assert all(l >= 0 for l in lats)
+    if inflight > 1 and budget > (1 << 30):
+        assert st.maxInflightSeen > 1
+    ref = _alone(torch, W, templates, picks)
+    for i, (a, b) in enumerate(zip(outs, ref)):
+        assert torch.equal(a, b), (i, templates[picks[i]].op, templates[picks[i]].n)
+
+
+def test_submit_validation(env):
+    import ctypes
+
+    torch, _lib, W, templates = env
+    lib = _lib.load()
+    srv = ctypes.c_void_p()
+    assert lib.FastFss_hip_serverCreate(ctypes.byref(srv), 1 << 30, 4, 2, 1) == 0
+    t = templates[0]
+    out = torch.empty(t.out_elems(), dtype=W.torch_dtype(t.es), device="cuda:0")
+    r = W.serve_request(t, out)
+    r.partyId = 5
+    tk = ctypes.c_uint64()
+    assert lib.FastFss_hip_serverSubmit(srv, ctypes.byref(r), ctypes.byref(tk)) == -12
+    r.partyId = 0
+    r.keyDataSize += 1
+    assert lib.FastFss_hip_serverSubmit(srv, ctypes.byref(r), ctypes.byref(tk)) == -2
+    r.keyDataSize -= 1
+    assert lib.FastFss_hip_serverSubmit(srv, ctypes.byref(r), ctypes.byref(tk)) == 0
+    assert lib.FastFss_hip_serverWait(srv, tk.value, None) == 0
+    assert lib.FastFss_hip_serverQuery(srv, tk.value) == 1
+    lib.FastFss_hip_serverDestroy(srv)
