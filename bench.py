@@ -33,6 +33,18 @@ OPS_PER_LEVEL = 10 * 12 + 2 * 10 * 32
 # HBM bytes per eval at n=32, u32: key 652 (sCW 512 + vCW 128 + tLCW, tRCW,
 # lastCW 12) + seed 16 + masked x 4 + share out 4.
 BYTES_PER_EVAL_N32 = 652 + 16 + 4 + 4
+# HBM bytes per launch of the eval kernel at 2^24 elements, from the two
+# calibrated rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) recorded here
+TRAFFIC_PROFILE = REPO / "profiles" / "dcf_eval_burst_n32_2p24_traffic.md"
+
+
+def profiled_traffic(log2n: int):
+    if log2n != 24 or not TRAFFIC_PROFILE.exists():
+        return None
+    for line in TRAFFIC_PROFILE.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":", 1)[1])
+    return None
 
 
 def parse():
@@ -278,7 +290,8 @@ def main():
             "peak": VALU_PEAK_TOPS,
             "unit": "TOP/s",
             "frac": achieved_tops / VALU_PEAK_TOPS,
-            "traffic": None,
+            "traffic": profiled_traffic(a.log2n),
+            "traffic_unit": "bytes/launch (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0, 1 GiB copy calibration)",
             "kernel": "dcf_eval_burst_kernel<uint32_t, 8, 1>",
             "kernel_ms": kernel_ms,
             "ops_per_eval": n * OPS_PER_LEVEL,
