@@ -30,6 +30,8 @@ HBM_PEAK_GBS = 8000.0
 # per level one AES-128 key schedule (10 rounds x 12 int ops) and two blocks
 # (10 rounds x 32 int ops: 16 byte extracts + 16 XORs on T-table lookups).
 OPS_PER_LEVEL = 10 * 12 + 2 * 10 * 32
+# DPF / Grotto per-point level: one key schedule and one block.
+DPF_OPS_PER_LEVEL = 10 * 12 + 10 * 32
 # HBM bytes per eval at n=32, u32: key 652 (sCW 512 + vCW 128 + tLCW, tRCW,
 # lastCW 12) + seed 16 + masked x 4 + share out 4.
 BYTES_PER_EVAL_N32 = 652 + 16 + 4 + 4
@@ -60,6 +62,8 @@ def parse():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--serve-qps", type=float, default=400.0)
     ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--no-dpf", action="store_true", help="skip the plain DPF leg")
+    ap.add_argument("--no-piecewise", action="store_true", help="skip the MIC + spline + full-domain Grotto leg")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after timing, run one device-to-device copy of this many MiB (counter calibration)")
     return ap.parse_args()
@@ -107,7 +111,7 @@ def ragged_leg(groups_n: int, reps: int = 5):
     persistent batched launch replayed from a hipGraph.  Party 0, keys resident."""
     import torch
 
-    from fastfunctionsecretsharing_amd import _lib, workloads as W
+    from fastfunctionsecretsharing_amd import _lib, autotune, workloads as W
 
     lib = _lib.load()
     groups = W.ragged_batch(groups=groups_n, seed=11)
@@ -120,6 +124,9 @@ def ragged_leg(groups_n: int, reps: int = 5):
     with torch.cuda.stream(stream):
         W.eval_plain(groups, 0, ref, sp)  # warm-up and reference
     stream.synchronize()
+    # tile sizes per (op, element size) picked by the autotuner on this batch
+    with torch.cuda.stream(stream):
+        tiles, _, tune_s = autotune.tune_batch_tiles(groups, party=0)
     desc = W.batch_descriptors(groups, 0, outs)
     plan = ctypes.c_void_p()
     assert lib.FastFss_hip_batchPlanCreate(ctypes.byref(plan), desc, len(groups)) == 0
@@ -159,7 +166,99 @@ def ragged_leg(groups_n: int, reps: int = 5):
         "plain_device_ms": plain_dev_ms,
         "batched_equals_plain": same,
         "groups_per_width": widths,
+        "autotuned_tiles": {"%s/es%d" % ("dcf" if op == 0 else "dpf", es): t for (op, es), t in sorted(tiles.items())},
+        "autotune_s": tune_s,
     }
+
+
+def _timed(stream, fn, reps):
+    import torch
+
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    stream.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    stream.synchronize()
+    return (time.perf_counter() - t0) / reps, e0.elapsed_time(e1) / reps
+
+
+def dpf_leg(log2n: int, reps: int = 10):
+    """Plain DPF eval, bitWidthIn = bitWidthOut = 32, 2^log2n elements, party 0, keys resident."""
+    import torch
+
+    from fastfunctionsecretsharing_amd import _lib, workloads as W
+
+    lib = _lib.load()
+    dev = torch.device("cuda:0")
+    N, n = 1 << log2n, 32
+    gen = torch.Generator(device=dev).manual_seed(77)
+    holder, sp = W.stream_ptr()
+    g = W.make_group(_lib.BATCH_DPF_EVAL, n, N, gen, dev, sp)
+    out = torch.empty(N, dtype=torch.int32, device=dev)
+
+    def step():
+        rc = lib.FastFss_cuda_dpfEval(out.data_ptr(), 4 * N, g.masked.data_ptr(), 4 * N, g.key.data_ptr(),
+                                      g.key_size, g.seeds[0].data_ptr(), 16 * N, 0, n, n, 1, 4, N, None, 0, sp)
+        assert rc == 0, rc
+
+    step()
+    wall, dev_ms = _timed(torch.cuda.current_stream(), step, reps)
+    ops = (n + 1) * DPF_OPS_PER_LEVEL  # n levels + the value conversion's AES (bitWidthOut > 16)
+    achieved = ops * N / (dev_ms * 1e-3) / 1e12
+    del holder
+    return {"workload": "plain DPF eval, bitWidthIn=32, bitWidthOut=32, groupSize=1, 2^%d elements, party 0" % log2n,
+            "evals_per_s": N / wall, "kernel_ms": dev_ms, "ops_per_eval": ops, "valu_tops": achieved,
+            "valu_frac": achieved / VALU_PEAK_TOPS}
+
+
+def piecewise_leg(reps: int = 3):
+    """DCF-MIC and a degree-2 spline, 16 intervals, bitWidthIn=24 / bitWidthOut=32, 2^22 elements;
+    full-domain Grotto at bitWidthIn=16 over 4096 keys (2^28 points).  Party 0, keys resident."""
+    import torch
+
+    from fastfunctionsecretsharing_amd import _lib, workloads as W
+
+    res = {}
+    stream = torch.cuda.current_stream()
+    holder, sp = W.stream_ptr()
+    w = W.piecewise_workload()
+    N, I = w.N, w.I
+    out = torch.empty(N * I, dtype=torch.int32, device="cuda:0")
+    W.mic_eval(w, 0, out, w.z, sp)
+    wall, dev_ms = _timed(stream, lambda: W.mic_eval(w, 0, out, w.z, sp), reps)
+    ops = (I + 1) * w.n * OPS_PER_LEVEL  # contiguous intervals share endpoints: I + 1 DCF evals per element
+    res["mic"] = {"workload": "DCF-MIC, 16 contiguous intervals, bitWidthIn=24, bitWidthOut=32, 2^22 elements",
+                  "elements_per_s": N / wall, "interval_evals_per_s": N * I / wall, "kernel_ms": dev_ms,
+                  "valu_frac": ops * N / (dev_ms * 1e-3) / 1e12 / VALU_PEAK_TOPS}
+    del out
+    cache = W.spline_cache(w)
+    out = torch.empty(N, dtype=torch.int32, device="cuda:0")
+    W.spline_eval(w, 0, out, w.e, w.beta, cache, sp)
+    wall, dev_ms = _timed(stream, lambda: W.spline_eval(w, 0, out, w.e, w.beta, cache, sp), reps)
+    res["spline"] = {"workload": "DCF-spline degree 2, 16 contiguous intervals, bitWidthIn=24, bitWidthOut=32, "
+                                 "2^22 elements, int32 coefficients",
+                     "elements_per_s": N / wall, "kernel_ms": dev_ms, "key_bytes": w.spl_ks}
+    del w, out, cache
+    torch.cuda.empty_cache()
+    gs = W.grotto_keys()
+    lv = gs.n - 6
+    pts = gs.N << gs.n
+    ops = gs.N * ((1 << lv) - 1) * OPS_PER_LEVEL  # one key schedule + two blocks per inner node
+    full = {"workload": "full-domain Grotto [x < alpha] shares, bitWidthIn=16, 4096 keys x 2^16 points, party 0"}
+    for name, packed, out in (("elements", 0, torch.empty(pts, dtype=torch.int16, device="cuda:0")),
+                              ("packed_bits", 1, torch.empty(pts // 64, dtype=torch.int64, device="cuda:0"))):
+        W.grotto_full(gs, 0, _lib.GROTTO_FULL_LT, packed, out, sp)
+        wall, dev_ms = _timed(stream, lambda: W.grotto_full(gs, 0, _lib.GROTTO_FULL_LT, packed, out, sp), reps * 3)
+        full[name] = {"points_per_s": pts / wall, "kernel_ms": dev_ms,
+                      "valu_frac": ops / (dev_ms * 1e-3) / 1e12 / VALU_PEAK_TOPS,
+                      "out_gbs": out.numel() * out.element_size() / (dev_ms * 1e-3) / 1e9}
+        del out
+    res["grotto_full_domain"] = full
+    del holder
+    return res
 
 
 def serving_leg(qps: float, requests: int):
@@ -299,9 +398,15 @@ def main():
             "hbm_frac": hbm_gbs / HBM_PEAK_GBS,
         },
     }
-    if not a.no_ragged:
-        del key, out
+    del key, out
+    torch.cuda.empty_cache()
+    if not a.no_dpf:
+        res["dpf"] = dpf_leg(a.log2n)
         torch.cuda.empty_cache()
+    if not a.no_piecewise:
+        res["piecewise"] = piecewise_leg()
+        torch.cuda.empty_cache()
+    if not a.no_ragged:
         res["ragged"] = ragged_leg(a.ragged_groups)
     if not a.no_serve:
         torch.cuda.empty_cache()

@@ -81,6 +81,8 @@ SIGNATURES = {
     "FastFss_hip_setEvalVariant": ("i", "int"),
     "FastFss_hip_getEvalVariantCount": ("", "int"),
     "FastFss_hip_setEvalVariantForElementSize": ("si", "int"),
+    "FastFss_hip_grottoEvalFullDomain": ("pspsps" + "iii" + "sss" + "p", "int"),
+    "FastFss_hip_grottoFullSetLaneTarget": ("s", "int"),
     "FastFss_hip_serverCreate": ("psiii", "int"),
     "FastFss_hip_serverSubmit": ("ppp", "int"),
     "FastFss_hip_serverQuery": ("pq", "int"),
@@ -147,6 +149,8 @@ class BatchGroup(ctypes.Structure):
         ("tileElems", ctypes.c_uint32),
     ]
 
+
+GROTTO_FULL_LT, GROTTO_FULL_LE, GROTTO_FULL_EQ = 0, 1, 2
 
 BATCH_DCF_EVAL = 0
 BATCH_DPF_EVAL = 1

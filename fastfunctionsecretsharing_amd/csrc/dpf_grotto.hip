@@ -6,8 +6,11 @@
 // items (elements, or element x point pairs for the multi-point forms).
 #include "FastFss/cuda/dpf.h"
 #include "FastFss/cuda/grotto.h"
+#include "FastFss/hip/grotto_full.h"
 #include "dpf_device.h"
 #include "fss_host.h"
+
+#include <atomic>
 
 namespace fss {
 
@@ -217,6 +220,195 @@ __global__ __launch_bounds__(kBlock) void grotto_lut_kernel(T *outE, T *outT, T 
             outT[e * lutNum + j] = party ? (T)(0 - acc) : acc;
         }
     }
+}
+
+// ------------------------------------------------------ full-domain Grotto ----
+//
+// Lane (e, r) owns the subtree r (0 <= r < 2^L) of key e: it walks the L top
+// levels along r (one key schedule + one block per level, as the per-point
+// eval does), then expands the remaining D = n-6-L levels depth-first, each
+// inner node giving both children from one key schedule and two blocks.  The
+// pending right children sit in a register stack; every lane of a wave visits
+// its leaves in the same order, so the stack depth is wave-uniform and the
+// stack slots are selected with static indices.
+//
+// A leaf covers 64 points.  With u the corrected leaf word and P the parity of
+// the path (src/impl/grotto.h:338-389), the shares of point p are
+//   eq: bit p of u
+//   lt: P ^ party ^ (bits 0..p of u)        (inclusive prefix parity)
+//   le: P ^ party ^ (bits 0..p-1 of u)      (exclusive prefix parity)
+// The per-point eval's early exit for le (remaining input bits all zero) only
+// skips left turns, which leave P unchanged, so it yields the same value.
+
+constexpr int kFullMaxDepth = 8;
+
+__device__ __forceinline__ uint64_t prefix_parity64(uint64_t u)
+{
+    u ^= u << 1;
+    u ^= u << 2;
+    u ^= u << 4;
+    u ^= u << 8;
+    u ^= u << 16;
+    u ^= u << 32;
+    return u;
+}
+
+// The 16-byte store q of a leaf's 64 group elements (0/1 values from `bits`).
+template <typename T>
+__device__ __forceinline__ void full_leaf_chunk(uint32_t w[4], uint64_t bits, int q)
+{
+    constexpr int per = 16 / (int)sizeof(T); // points per 16-byte store
+    const uint64_t b  = bits >> (q * per);
+    if constexpr (sizeof(T) == 1)
+    {
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+        {
+            const uint32_t nib = (uint32_t)(b >> (4 * j)) & 0xFu;
+            w[j] = (nib & 1u) | ((nib & 2u) << 7) | ((nib & 4u) << 14) | ((nib & 8u) << 21);
+        }
+    }
+    else if constexpr (sizeof(T) == 2)
+    {
+#pragma unroll
+        for (int j = 0; j < 4; j++) w[j] = ((uint32_t)(b >> (2 * j)) & 1u) | (((uint32_t)(b >> (2 * j + 1)) & 1u) << 16);
+    }
+    else if constexpr (sizeof(T) == 4)
+    {
+#pragma unroll
+        for (int j = 0; j < 4; j++) w[j] = (uint32_t)(b >> j) & 1u;
+    }
+    else
+    {
+        w[0] = (uint32_t)b & 1u;
+        w[1] = 0;
+        w[2] = (uint32_t)(b >> 1) & 1u;
+        w[3] = 0;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void grotto_full_kernel(void *out, const uint8_t *key, const uint8_t *seed,
+                                                             int party, int mode, int packed, int n, int L, int D,
+                                                             size_t N)
+{
+    FSS_KERNEL_PROLOGUE
+    const GrottoKeyPtrs k     = grotto_key(key, n, sizeof(T), N);
+    const T            *tlr   = (const T *)k.tlr;
+    const size_t        lanes = N << L;
+    const int           lv    = n - 6; // tree levels above the leaf words
+    FSS_GRID_LOOP(w, lanes)
+    {
+        const size_t   e  = w >> L;
+        const uint32_t r  = (uint32_t)(w & ((1u << L) - 1u));
+        const uint8_t *sc = k.sCW + e * 16 * (size_t)lv;
+        const T        tl = tlr[2 * e], tr = tlr[2 * e + 1];
+        const uint64_t lastCW = k.lastCW[e];
+        uint32_t       s[4];
+        load16(s, seed + 16 * e);
+        s[0] &= ~1u;
+        uint32_t t = (uint32_t)party, P = 0;
+        for (int i = 0; i < L; i++)
+        {
+            const uint32_t bit  = (r >> (L - 1 - i)) & 1u;
+            const uint32_t prev = t;
+            tree_eval_step<T>(c, s, t, bit, sc + 16 * i, tl, tr, i);
+            P ^= bit & (prev ^ t);
+        }
+
+        uint32_t stS[kFullMaxDepth][4];
+        uint32_t stTP[kFullMaxDepth]; // t | parity << 1 of the pending right child
+        const uint32_t leaves = 1u << D;
+        int            depth  = 0;
+        for (uint32_t leaf = 0;;)
+        {
+            while (depth < D)
+            {
+                const int level = L + depth;
+                uint32_t  blk[2][4] = {{0u, 0u, 1u, 0u}, {2u, 0u, 3u, 0u}};
+                uint32_t  cw[4];
+                load16(cw, sc + 16 * level);
+                aes_encrypt<2>(c, s, blk);
+                const uint32_t mk = 0u - t;
+                const uint32_t tL = (blk[0][0] & 1u) ^ ((uint32_t)(tl >> level) & 1u & t);
+                const uint32_t tR = (blk[1][0] & 1u) ^ ((uint32_t)(tr >> level) & 1u & t);
+                const uint32_t rs0 = (blk[1][0] & ~1u) ^ (cw[0] & mk), rs1 = blk[1][1] ^ (cw[1] & mk);
+                const uint32_t rs2 = blk[1][2] ^ (cw[2] & mk), rs3 = blk[1][3] ^ (cw[3] & mk);
+                const uint32_t rtp = tR | ((P ^ t ^ tR) << 1);
+#pragma unroll
+                for (int d = 0; d < kFullMaxDepth; d++)
+                    if (d == depth)
+                    {
+                        stS[d][0] = rs0;
+                        stS[d][1] = rs1;
+                        stS[d][2] = rs2;
+                        stS[d][3] = rs3;
+                        stTP[d]   = rtp;
+                    }
+                s[0] = (blk[0][0] & ~1u) ^ (cw[0] & mk);
+                s[1] = blk[0][1] ^ (cw[1] & mk);
+                s[2] = blk[0][2] ^ (cw[2] & mk);
+                s[3] = blk[0][3] ^ (cw[3] & mk);
+                t    = tL;
+                depth++;
+            }
+
+            const uint64_t u = ((uint64_t)s[2] | ((uint64_t)s[3] << 32)) ^ (t ? lastCW : 0ull);
+            uint64_t       bits;
+            if (mode == 2)
+                bits = u;
+            else
+            {
+                bits = prefix_parity64(mode == 1 ? (u << 1) : u);
+                if ((P ^ (uint32_t)party) & 1u) bits = ~bits;
+            }
+            const size_t leafIdx = ((size_t)r << D) | leaf; // leaf word index within key e
+            if (packed)
+                ((uint64_t *)out)[(e << lv) + leafIdx] = bits;
+            else
+            {
+                uint8_t *dst = (uint8_t *)out + (((e << lv) + leafIdx) << 6) * sizeof(T);
+#pragma unroll
+                for (int q = 0; q < 4 * (int)sizeof(T); q++)
+                {
+                    uint32_t v[4];
+                    full_leaf_chunk<T>(v, bits, q);
+                    store16(dst + 16 * q, v);
+                }
+            }
+
+            if (++leaf == leaves) break;
+            // the next leaf's path leaves the current one at depth D-1-ctz(leaf):
+            // resume from the right child pushed there
+            const int d0 = D - 1 - (int)__builtin_ctz(leaf);
+#pragma unroll
+            for (int d = 0; d < kFullMaxDepth; d++)
+                if (d == d0)
+                {
+                    s[0] = stS[d][0];
+                    s[1] = stS[d][1];
+                    s[2] = stS[d][2];
+                    s[3] = stS[d][3];
+                    t    = stTP[d] & 1u;
+                    P    = stTP[d] >> 1;
+                }
+            depth = d0 + 1;
+        }
+    }
+}
+
+// Subtree split: enough lanes to fill every CU once (or the configured
+// target), and at most kFullMaxDepth levels expanded per lane.
+static std::atomic<size_t> g_full_lane_target{0};
+
+inline void grotto_full_split(int lv, size_t N, int &L, int &D)
+{
+    const size_t conf   = g_full_lane_target.load(std::memory_order_relaxed);
+    const size_t target = conf ? conf : (size_t)device_cu_count() * kBlock;
+    L                   = 0;
+    while (L < lv && (N << L) < target) L++;
+    if (lv - L > kFullMaxDepth) L = lv - kFullMaxDepth;
+    D = lv - L;
 }
 
 } // namespace fss
@@ -485,4 +677,40 @@ extern "C" FAST_FSS_API int FastFss_cuda_grottoIntervalLutEval(
     int rc = launch_status();
     hipFreeAsync(bits, st);
     return rc;
+}
+
+// ------------------------------------------------- C ABI: full-domain Grotto ----
+
+extern "C" FAST_FSS_API int FastFss_hip_grottoEvalFullDomain(void *sharedOut, size_t sharedOutDataSize,
+                                                             const void *key, size_t keyDataSize, const void *seed,
+                                                             size_t seedDataSize, int partyId, int mode, int packed,
+                                                             size_t bitWidthIn, size_t elementSize,
+                                                             size_t elementNum, void *hipStreamPtr)
+{
+    const size_t n = bitWidthIn, es = elementSize, N = elementNum;
+    if (!elem_size_ok(es)) return FAST_FSS_INVALID_ELEMENT_SIZE_ERROR;
+    if (!(6 <= n && n <= es * 8 && n <= 40)) return FAST_FSS_INVALID_BITWIDTH_ERROR;
+    if (mode < FAST_FSS_GROTTO_FULL_LT || mode > FAST_FSS_GROTTO_FULL_EQ || (packed != 0 && packed != 1))
+        return FAST_FSS_RUNTIME_ERROR;
+    if (partyId != 0 && partyId != 1) return FAST_FSS_INVALID_PARTY_ID_ERROR;
+    if (keyDataSize != grotto_key_size(n, es, N)) return FAST_FSS_INVALID_KEY_DATA_SIZE_ERROR;
+    if (seedDataSize != N * 16) return FAST_FSS_INVALID_SEED_DATA_SIZE_ERROR;
+    const size_t perKey = packed ? ((size_t)8 << (n - 6)) : (es << n);
+    if (N != 0 && perKey > SIZE_MAX / N) return FAST_FSS_INVALID_SHARED_OUT_DATA_SIZE_ERROR;
+    if (sharedOutDataSize != N * perKey) return FAST_FSS_INVALID_SHARED_OUT_DATA_SIZE_ERROR;
+    if (N == 0) return FAST_FSS_SUCCESS;
+    if (!aligned(key, 8) || !aligned(seed, 8) || !aligned(sharedOut, 16)) return FAST_FSS_RUNTIME_ERROR;
+    int L, D;
+    grotto_full_split((int)n - 6, N, L, D);
+    hipStream_t st = resolve_stream(hipStreamPtr);
+    FSS_DISPATCH(es, grotto_full_kernel<T><<<grid_for(N << L), kBlock, 0, st>>>(
+                         sharedOut, (const uint8_t *)key, (const uint8_t *)seed, partyId, mode, packed, (int)n, L, D,
+                         N));
+    return launch_status();
+}
+
+extern "C" FAST_FSS_API int FastFss_hip_grottoFullSetLaneTarget(size_t lanes)
+{
+    g_full_lane_target.store(lanes, std::memory_order_relaxed);
+    return FAST_FSS_SUCCESS;
 }

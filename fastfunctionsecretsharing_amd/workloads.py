@@ -346,3 +346,158 @@ def run_trace(templates, picks, arrivals, budget, max_inflight, streams, launch_
     lib.FastFss_hip_serverStats(srv, ctypes.byref(st))
     lib.FastFss_hip_serverDestroy(srv)
     return outs, lats, wall, st
+
+
+# ---------------------------------------------------------------------------
+# MIC + spline batch and full-domain Grotto: DCF-MIC and a piecewise degree-2
+# spline with 16 intervals at bitWidthIn = 24 (32-bit elements), bitWidthOut =
+# 32, 2^22 elements; Grotto at bitWidthIn = 16 over 4096 keys, every point of
+# the 2^16 domain.  Endpoints are uniform sorted contiguous intervals covering
+# [0, 2^24); coefficients are uniform int32 (fixed point with 12 fractional
+# bits only scales them).
+# ---------------------------------------------------------------------------
+
+
+@dataclass
+class PiecewiseSet:
+    n: int
+    m: int
+    es: int
+    N: int
+    I: int
+    d: int
+    alpha: object
+    x: object
+    masked: object
+    seeds: object
+    left: object
+    right: object
+    mic_key: object = None
+    mic_ks: int = 0
+    z: object = None
+    spl_key: object = None
+    spl_ks: int = 0
+    coeff: object = None
+    e: object = None
+    beta: object = None
+
+
+def piecewise_workload(N=1 << 22, n=24, m=32, I=16, d=2, seed=4, mic=True, spline=True, device="cuda:0"):
+    """Resident MIC and spline key sets on one set of inputs (keys from the GPU keyGen entry points)."""
+    import torch
+
+    lib = _lib.load()
+    dev = torch.device(device)
+    es = 4
+    dt = torch_dtype(es)
+    rng = np.random.default_rng(seed)
+    gen = torch.Generator(device=dev).manual_seed(seed)
+    holder, sp = stream_ptr()
+    alpha64 = _rand_bits(gen, N, n, dev)
+    x64 = _rand_bits(gen, N, n, dev)
+    w = PiecewiseSet(n, m, es, N, I, d, alpha64.to(dt), x64.to(dt), ((x64 + alpha64) & ((1 << n) - 1)).to(dt),
+                     torch.randint(0, 256, (2, 16 * N), dtype=torch.uint8, device=dev, generator=gen), None, None)
+    left, right = _sorted_intervals(rng, I, n)
+    w.left = torch.tensor(left, dtype=torch.int64).to(dt).to(dev)
+    w.right = torch.tensor(right, dtype=torch.int64).to(dt).to(dev)
+    ks = ctypes.c_size_t(0)
+    s0, s1 = w.seeds[0].data_ptr(), w.seeds[1].data_ptr()
+    if mic:
+        assert lib.FastFss_dcfMICGetKeyDataSize(ctypes.addressof(ks), n, m, es, N) == 0
+        w.mic_ks = ks.value
+        w.mic_key = torch.empty(ks.value, dtype=torch.uint8, device=dev)
+        w.z = torch.empty(N * I, dtype=dt, device=dev)
+        rc = lib.FastFss_cuda_dcfMICKeyGen(w.mic_key.data_ptr(), ks.value, w.z.data_ptr(), N * I * es,
+                                           w.alpha.data_ptr(), es * N, s0, 16 * N, s1, 16 * N, w.left.data_ptr(),
+                                           es * I, w.right.data_ptr(), es * I, n, m, es, N, sp)
+        assert rc == 0, rc
+    if spline:
+        g = I * (d + 1)
+        w.coeff = torch.randint(-(2**31), 2**31 - 1, (g,), dtype=torch.int64, device=dev, generator=gen).to(dt)
+        assert lib.FastFss_dcfSplineGetKeyDataSize(ctypes.addressof(ks), d, I, n, m, es, N) == 0
+        w.spl_ks = ks.value
+        w.spl_key = torch.empty(ks.value, dtype=torch.uint8, device=dev)
+        w.e = torch.empty(N * g, dtype=dt, device=dev)
+        w.beta = torch.empty(N * g, dtype=dt, device=dev)
+        rc = lib.FastFss_cuda_dcfSplineKeyGen(w.spl_key.data_ptr(), ks.value, w.e.data_ptr(), N * g * es,
+                                              w.beta.data_ptr(), N * g * es, w.alpha.data_ptr(), es * N, s0, 16 * N,
+                                              s1, 16 * N, w.coeff.data_ptr(), g * es, d, w.left.data_ptr(), es * I,
+                                              w.right.data_ptr(), es * I, I, n, m, es, N, sp)
+        assert rc == 0, rc
+    torch.cuda.synchronize()
+    del holder
+    return w
+
+
+def mic_eval(w, party, out, z_share, sp):
+    lib = _lib.load()
+    N, I, es = w.N, w.I, w.es
+    rc = lib.FastFss_cuda_dcfMICEval(out.data_ptr(), es * N * I, w.masked.data_ptr(), es * N, w.mic_key.data_ptr(),
+                                     w.mic_ks, z_share.data_ptr(), es * N * I, w.seeds[party].data_ptr(), 16 * N,
+                                     party, w.left.data_ptr(), es * I, w.right.data_ptr(), es * I, w.n, w.m, es, N,
+                                     None, 0, sp)
+    if rc != 0:
+        raise RuntimeError(f"FastFss_cuda_dcfMICEval failed : {rc}")
+
+
+def spline_cache(w):
+    import torch
+
+    lib = _lib.load()
+    cs = ctypes.c_size_t(0)
+    assert lib.FastFss_dcfSplineGetCacheDataSize(ctypes.addressof(cs), w.d, w.I, w.n, w.m, w.es, w.N) == 0
+    return torch.empty(cs.value, dtype=torch.uint8, device=w.spl_key.device)
+
+
+def spline_eval(w, party, out, e_share, beta_share, cache, sp):
+    lib = _lib.load()
+    N, I, es, d = w.N, w.I, w.es, w.d
+    g = I * (d + 1)
+    rc = lib.FastFss_cuda_dcfSplineEval(out.data_ptr(), es * N, w.masked.data_ptr(), es * N, w.spl_key.data_ptr(),
+                                        w.spl_ks, e_share.data_ptr(), es * N * g, beta_share.data_ptr(), es * N * g,
+                                        w.seeds[party].data_ptr(), 16 * N, party, w.left.data_ptr(), es * I,
+                                        w.right.data_ptr(), es * I, I, d, w.n, w.m, es, N, cache.data_ptr(),
+                                        cache.numel(), sp)
+    if rc != 0:
+        raise RuntimeError(f"FastFss_cuda_dcfSplineEval failed : {rc}")
+
+
+@dataclass
+class GrottoSet:
+    n: int
+    es: int
+    N: int
+    alpha: object
+    seeds: object
+    key: object
+    ks: int
+
+
+def grotto_keys(N=4096, n=16, seed=6, device="cuda:0"):
+    import torch
+
+    lib = _lib.load()
+    dev = torch.device(device)
+    es = 2 if n <= 16 else (4 if n <= 32 else 8)
+    gen = torch.Generator(device=dev).manual_seed(seed)
+    holder, sp = stream_ptr()
+    alpha = _rand_bits(gen, N, n, dev).to(torch_dtype(es))
+    seeds = torch.randint(0, 256, (2, 16 * N), dtype=torch.uint8, device=dev, generator=gen)
+    ks = ctypes.c_size_t(0)
+    assert lib.FastFss_grottoGetKeyDataSize(ctypes.addressof(ks), n, es, N) == 0
+    key = torch.empty(ks.value, dtype=torch.uint8, device=dev)
+    rc = lib.FastFss_cuda_grottoKeyGen(key.data_ptr(), ks.value, alpha.data_ptr(), es * N, seeds[0].data_ptr(),
+                                       16 * N, seeds[1].data_ptr(), 16 * N, n, es, N, sp)
+    assert rc == 0, rc
+    torch.cuda.synchronize()
+    del holder
+    return GrottoSet(n, es, N, alpha, seeds, key, ks.value)
+
+
+def grotto_full(gs, party, mode, packed, out, sp):
+    lib = _lib.load()
+    rc = lib.FastFss_hip_grottoEvalFullDomain(out.data_ptr(), out.numel() * out.element_size(), gs.key.data_ptr(),
+                                              gs.ks, gs.seeds[party].data_ptr(), 16 * gs.N, party, mode, packed,
+                                              gs.n, gs.es, gs.N, sp)
+    if rc != 0:
+        raise RuntimeError(f"FastFss_hip_grottoEvalFullDomain failed : {rc}")
