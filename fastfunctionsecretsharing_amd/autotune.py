@@ -65,7 +65,10 @@ def tune_batch_tiles(groups, party=0, candidates=TILE_CANDIDATES, reps=3, sweeps
 
     lib = _lib.load()
     t0 = time.perf_counter()
-    stream = torch.cuda.current_stream()
+    # graph capture needs a stream other than the legacy default stream
+    caller = torch.cuda.current_stream()
+    stream = torch.cuda.Stream()
+    stream.wait_stream(caller)
     holder, sp = W.stream_ptr(stream)
     outs = W.alloc_outputs(groups)
     desc = W.batch_descriptors(groups, party, outs)
@@ -88,6 +91,7 @@ def tune_batch_tiles(groups, party=0, candidates=TILE_CANDIDATES, reps=3, sweeps
                 t = run(trial)
                 if t < best * (1.0 - min_gain):
                     best, table = t, trial
+    caller.wait_stream(stream)
     if apply:
         for (op, es), tile in table.items():
             lib.FastFss_hip_batchSetDefaultTile(op, es, tile)
