@@ -13,6 +13,27 @@
 namespace fss {
 
 template <typename T>
+__global__ __launch_bounds__(kBlock) void dcf_keygen_g1_kernel(uint8_t *key, const T *alpha, const T *beta,
+                                                               const uint8_t *seed0, const uint8_t *seed1, int n,
+                                                               int m, size_t N)
+{
+    __shared__ uint32_t lds[kLdsTableWords];
+    aes_fill_lds(lds);
+    __syncthreads();
+    const AesCtx c      = aes_ctx(lds);
+    const size_t stride = (size_t)n + 3;
+    uint8_t     *tails  = key + N * 16 * (size_t)n;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < N; e += (size_t)gridDim.x * blockDim.x)
+    {
+        uint32_t a[4], b[4];
+        load16(a, seed0 + 16 * e);
+        load16(b, seed1 + 16 * e);
+        dcf_keygen_g1<T, 4>(c, key + e * 16 * (size_t)n, (T *)tails + e * stride, alpha[e], beta ? beta[e] : (T)1,
+                            a, b, n, m);
+    }
+}
+
+template <typename T>
 __global__ __launch_bounds__(kBlock) void dcf_keygen_kernel(uint8_t *key, const T *alpha, const T *beta,
                                                             const uint8_t *seed0, const uint8_t *seed1, int n, int m,
                                                             int g, size_t N)
@@ -208,9 +229,18 @@ extern "C" FAST_FSS_API int FastFss_cuda_dcfKeyGen(void *key, size_t keyDataSize
     if (!aligned(key, 8) || !aligned(seed0, 8) || !aligned(seed1, 8)) return FAST_FSS_RUNTIME_ERROR;
     hipStream_t st = resolve_stream(cudaStreamPtr);
     const void *b  = betaDataSize ? beta : nullptr;
-    FSS_DISPATCH(es, dcf_keygen_kernel<T><<<grid_for(N), kBlock, 0, st>>>(
-                         (uint8_t *)key, (const T *)alpha, (const T *)b, (const uint8_t *)seed0,
-                         (const uint8_t *)seed1, (int)n, (int)m, (int)g, N));
+    if (g == 1)
+    {
+        FSS_DISPATCH(es, dcf_keygen_g1_kernel<T><<<grid_for(N), kBlock, 0, st>>>(
+                             (uint8_t *)key, (const T *)alpha, (const T *)b, (const uint8_t *)seed0,
+                             (const uint8_t *)seed1, (int)n, (int)m, N));
+    }
+    else
+    {
+        FSS_DISPATCH(es, dcf_keygen_kernel<T><<<grid_for(N), kBlock, 0, st>>>(
+                             (uint8_t *)key, (const T *)alpha, (const T *)b, (const uint8_t *)seed0,
+                             (const uint8_t *)seed1, (int)n, (int)m, (int)g, N));
+    }
     return launch_status();
 }
 

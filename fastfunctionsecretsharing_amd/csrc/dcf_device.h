@@ -256,6 +256,89 @@ __device__ void dcf_eval_global(const AesCtx &c, T *out, const uint8_t *sCW, con
 // Key generation for one element (reference src/impl/dcf.h:187-368).
 // `beta` == nullptr means beta = (1, 0, ...).  lastCW doubles as the running
 // vAlpha accumulator, as in the reference.
+// groupSize-1 keyGen (bitWidthOut <= 64, so the value conversion is the low
+// ceil(bitWidthOut/8) bytes of a block): the two parties' blocks are produced
+// one party at a time and only the values each level needs are kept, so the
+// walk stays in registers; the running lastCW correction is a register, and
+// the correction words of CH levels are stored in one burst per lane.
+// Same outputs as dcf_keygen_one with g = 1 (src/impl/dcf.h:187-368).
+template <typename T, int CH>
+__device__ void dcf_keygen_g1(const AesCtx &c, uint8_t *sCW, T *tail, T alpha, T beta, const uint32_t seed0[4],
+                              const uint32_t seed1[4], int n, int m)
+{
+    const int ob    = (m + 7) / 8;
+    uint32_t  s0[4] = {seed0[0] & ~1u, seed0[1], seed0[2], seed0[3]};
+    uint32_t  s1[4] = {seed1[0] & ~1u, seed1[1], seed1[2], seed1[3]};
+    uint32_t  t0 = 0, t1 = 1;
+    T         tlBits = 0, trBits = 0, last = 0;
+    for (int i0 = 0; i0 < n; i0 += CH)
+    {
+        uint32_t cwb[CH][4];
+        T        vcw[CH];
+#pragma unroll
+        for (int k = 0; k < CH; k++)
+        {
+            const int i = i0 + k;
+            if (i >= n) break;
+            const uint32_t keep = (uint32_t)(alpha >> (n - 1 - i)) & 1u;
+            uint32_t       b[4][4];
+            uint32_t       sk0[4], sl0[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) prg_pt(b[q], q);
+            aes_encrypt<4>(c, s0, b);
+            const uint32_t tL0 = b[0][0] & 1u, tR0 = b[2][0] & 1u;
+            b[0][0] &= ~1u, b[2][0] &= ~1u;
+#pragma unroll
+            for (int w = 0; w < 4; w++) sk0[w] = keep ? b[2][w] : b[0][w], sl0[w] = keep ? b[0][w] : b[2][w];
+            const T vl0 = (T)low_bytes(keep ? b[1] : b[3], ob), vk0 = (T)low_bytes(keep ? b[3] : b[1], ob);
+#pragma unroll
+            for (int q = 0; q < 4; q++) prg_pt(b[q], q);
+            aes_encrypt<4>(c, s1, b);
+            const uint32_t tL1 = b[0][0] & 1u, tR1 = b[2][0] & 1u;
+            b[0][0] &= ~1u, b[2][0] &= ~1u;
+            const T vl1 = (T)low_bytes(keep ? b[1] : b[3], ob), vk1 = (T)low_bytes(keep ? b[3] : b[1], ob);
+            uint32_t cw[4], sk1[4];
+#pragma unroll
+            for (int w = 0; w < 4; w++)
+            {
+                sk1[w] = keep ? b[2][w] : b[0][w];
+                cw[w]  = sl0[w] ^ (keep ? b[0][w] : b[2][w]);
+                cwb[k][w] = cw[w];
+            }
+            const T sign1 = t1 ? (T)-1 : (T)1;
+            T       v     = (T)((T)(vl1 - vl0 - last) * sign1);
+            if (keep) v = (T)(v + (T)(sign1 * beta)); // the lost child is the left one
+            vcw[k] = v;
+            last   = (T)(last - vk1 + vk0 + (T)(sign1 * v));
+            const uint32_t tLCW = tL0 ^ tL1 ^ keep ^ 1u;
+            const uint32_t tRCW = tR0 ^ tR1 ^ keep;
+            tlBits |= (T)((T)tLCW << i);
+            trBits |= (T)((T)tRCW << i);
+            const uint32_t m0 = 0u - t0, m1 = 0u - t1;
+#pragma unroll
+            for (int w = 0; w < 4; w++)
+            {
+                s0[w] = sk0[w] ^ (cw[w] & m0);
+                s1[w] = sk1[w] ^ (cw[w] & m1);
+            }
+            const uint32_t tk0 = keep ? tR0 : tL0, tk1 = keep ? tR1 : tL1, tkc = keep ? tRCW : tLCW;
+            t0 = tk0 ^ (t0 & tkc);
+            t1 = tk1 ^ (t1 & tkc);
+        }
+#pragma unroll
+        for (int k = 0; k < CH; k++)
+        {
+            if (i0 + k >= n) break;
+            store16(sCW + 16 * (i0 + k), cwb[k]);
+            tail[i0 + k] = vcw[k];
+        }
+    }
+    tail[n]       = tlBits;
+    tail[n + 1]   = trBits;
+    const T sign1 = t1 ? (T)-1 : (T)1;
+    tail[n + 2]   = (T)((T)((T)(0 - last) + (T)low_bytes(s1, ob) - (T)low_bytes(s0, ob)) * sign1);
+}
+
 template <typename T>
 __device__ void dcf_keygen_one(const AesCtx &c, uint8_t *sCW, T *tail, T alpha, const T *beta,
                                const uint32_t seed0[4], const uint32_t seed1[4], int n, int m, int g)

@@ -121,6 +121,60 @@ __device__ __forceinline__ void dpf_eval_one(const AesCtx &c, T *out, const uint
     }
 }
 
+// groupSize-1 DPF eval with the correction words of CH levels loaded in one
+// burst before they are used (one 16-byte load per level per lane otherwise
+// interleaves with the AES and lets L2 evict the lane's key line between
+// levels).  The value conversion for groupSize 1 takes one byte (the
+// reference's swapped ConvertCtx arguments, src/impl/dpf.h:74,278,283): byte 0
+// of the final seed, or of AES_seed(0^128) when bitWidthOut > 16.  Results
+// equal dpf_eval_one.
+template <typename T, int CH>
+__device__ __forceinline__ T dpf_eval_g1_burst(const AesCtx &c, const uint8_t *sCW, const T *tail, T xin,
+                                               uint32_t s[4], int party, int n, int m)
+{
+    s[0] &= ~1u;
+    uint32_t t  = (uint32_t)party;
+    const T  x  = mod_bits<T>(xin, n);
+    const T  tl = tail[0], tr = tail[1];
+    for (int i0 = 0; i0 < n; i0 += CH)
+    {
+        uint32_t cw[CH][4];
+#pragma unroll
+        for (int k = 0; k < CH; k++)
+            if (i0 + k < n) load16(cw[k], sCW + 16 * (i0 + k));
+#pragma unroll
+        for (int k = 0; k < CH; k++)
+        {
+            const int i = i0 + k;
+            if (i < n)
+            {
+                const uint32_t bit       = (uint32_t)(x >> (n - 1 - i)) & 1u;
+                uint32_t       blk[1][4] = {{2u * bit, 0u, 2u * bit + 1u, 0u}};
+                aes_encrypt<1>(c, s, blk);
+                const uint32_t mk = 0u - t;
+                uint32_t       tn = blk[0][0] & 1u;
+                s[0]              = (blk[0][0] & ~1u) ^ (cw[k][0] & mk);
+                s[1]              = blk[0][1] ^ (cw[k][1] & mk);
+                s[2]              = blk[0][2] ^ (cw[k][2] & mk);
+                s[3]              = blk[0][3] ^ (cw[k][3] & mk);
+                tn ^= (uint32_t)(((bit ? tr : tl) >> i) & 1) & t;
+                t = tn;
+            }
+        }
+    }
+    uint32_t conv;
+    if (m <= 16)
+        conv = s[0] & 0xffu;
+    else
+    {
+        uint32_t blk[1][4] = {{0u, 0u, 0u, 0u}};
+        aes_encrypt<1>(c, s, blk);
+        conv = blk[0][0] & 0xffu;
+    }
+    const T term = (T)((T)conv + (t ? tail[2] : (T)0));
+    return party ? (T)(0 - term) : term;
+}
+
 // ---- Grotto ----
 // key: [sCW N x (n-6) x 16][lastCW N x 8][tLCW, tRCW N x 2 GE] (src/impl/grotto.h:53-79)
 

@@ -58,6 +58,20 @@ __global__ __launch_bounds__(kBlock) void dpf_eval_kernel(T *out, const T *x, co
     }
 }
 
+template <typename T, int CH>
+__global__ __launch_bounds__(kBlock) void dpf_eval_burst_kernel(T *out, const T *x, const uint8_t *key,
+                                                                const uint8_t *seed, int party, int n, int m, size_t N)
+{
+    FSS_KERNEL_PROLOGUE
+    const T *tails = (const T *)(key + N * 16 * (size_t)n);
+    FSS_GRID_LOOP(e, N)
+    {
+        uint32_t s[4];
+        load16(s, seed + 16 * e);
+        out[e] = dpf_eval_g1_burst<T, CH>(c, key + e * 16 * (size_t)n, tails + e * 3, x[e], s, party, n, m);
+    }
+}
+
 // One work item per (element, point): point j of element e lands at
 // out[e][(x_e - j) mod 2^n] (src/kernel/dpf.h:253-276).
 template <typename T>
@@ -478,9 +492,19 @@ extern "C" FAST_FSS_API int FastFss_cuda_dpfEval(void *sharedOut, size_t sharedO
     if (N == 0) return FAST_FSS_SUCCESS;
     if (!aligned(key, 8) || !aligned(seed, 8)) return FAST_FSS_RUNTIME_ERROR;
     hipStream_t st = resolve_stream(cudaStreamPtr);
-    FSS_DISPATCH(es, dpf_eval_kernel<T><<<grid_for(N), kBlock, 0, st>>>((T *)sharedOut, (const T *)maskedX,
-                                                                         (const uint8_t *)key, (const uint8_t *)seed,
-                                                                         partyId, (int)n, (int)m, (int)g, N));
+    if (g == 1)
+    {
+        FSS_DISPATCH(es, dpf_eval_burst_kernel<T, 8><<<grid_for(N), kBlock, 0, st>>>(
+                             (T *)sharedOut, (const T *)maskedX, (const uint8_t *)key, (const uint8_t *)seed,
+                             partyId, (int)n, (int)m, N));
+    }
+    else
+    {
+        FSS_DISPATCH(es, dpf_eval_kernel<T><<<grid_for(N), kBlock, 0, st>>>((T *)sharedOut, (const T *)maskedX,
+                                                                             (const uint8_t *)key,
+                                                                             (const uint8_t *)seed, partyId, (int)n,
+                                                                             (int)m, (int)g, N));
+    }
     return launch_status();
 }
 

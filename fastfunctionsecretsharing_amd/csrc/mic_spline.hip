@@ -40,11 +40,9 @@ __global__ __launch_bounds__(kBlock) void mic_keygen_kernel(uint8_t *key, T *z, 
         uint32_t a[4], b[4];
         load16(a, seed0 + 16 * e);
         load16(b, seed1 + 16 * e);
-        const T al  = alpha[e];
-        const T one = 1;
+        const T al = alpha[e];
         // DCF on alpha - 1 with beta = 1 (src/impl/mic.h:22)
-        dcf_keygen_one<T>(c, key + e * 16 * (size_t)n, tails + e * (size_t)(n + 3), (T)(al - 1), &one, a, b, n, m,
-                          1);
+        dcf_keygen_g1<T, 4>(c, key + e * 16 * (size_t)n, tails + e * (size_t)(n + 3), (T)(al - 1), (T)1, a, b, n, m);
         for (size_t i = 0; i < I; i++)
         {
             const T qP = mod_bits<T>((T)(right[i] + 1), n);
@@ -72,16 +70,24 @@ __device__ __forceinline__ void mic_eval_range(const AesCtx &c, T *out, const T 
         const T xP  = mod_bits<T>((T)(x - 1 - left[i]), n);
         const T xQp = mod_bits<T>((T)(x - 1 - (T)(right[i] + 1)), n);
         const T p   = mod_bits<T>(left[i], n);
-        uint32_t s[4];
+        uint32_t       s[1][4];
+        const uint8_t *sc[1] = {sCW};
+        const T       *tl[1] = {tail};
         if (i > first && left[i] == mod_bits<T>((T)(right[i - 1] + 1), n))
             sp = sq;
         else
         {
-            s[0] = seed[0], s[1] = seed[1], s[2] = seed[2], s[3] = seed[3];
-            sp   = dcf_eval_g1<T>(c, sCW, tail, xP, s, party, n, outBytes);
+            s[0][0] = seed[0], s[0][1] = seed[1], s[0][2] = seed[2], s[0][3] = seed[3];
+            const T xin[1] = {xP};
+            T       r[1];
+            dcf_eval_g1_burst<T, 8, 1>(c, r, sc, tl, xin, s, party, n, outBytes);
+            sp = r[0];
         }
-        s[0] = seed[0], s[1] = seed[1], s[2] = seed[2], s[3] = seed[3];
-        sq  = dcf_eval_g1<T>(c, sCW, tail, xQp, s, party, n, outBytes);
+        s[0][0] = seed[0], s[0][1] = seed[1], s[0][2] = seed[2], s[0][3] = seed[3];
+        const T xin[1] = {xQp};
+        T       r[1];
+        dcf_eval_g1_burst<T, 8, 1>(c, r, sc, tl, xin, s, party, n, outBytes);
+        sq = r[0];
         T o = (T)(sq - sp + zs[i]);
         if (party == 1) o = (T)(o + (T)((int)(x > p) - (int)(x > qP)));
         out[i] = o;
