@@ -105,6 +105,20 @@ def cpu_baseline(log2n: int):
     }
 
 
+def ragged_ops(groups):
+    """Algorithmic VALU ops of one pass over a ragged batch: DCF n x 760, DPF n x 440
+    plus one more AES level for the value conversion when bitWidthOut > 16."""
+    from fastfunctionsecretsharing_amd import _lib
+
+    ops = 0
+    for g in groups:
+        if g.op == _lib.BATCH_DCF_EVAL:
+            ops += g.N * g.n * OPS_PER_LEVEL
+        else:
+            ops += g.N * (g.n + (1 if g.n > 16 else 0)) * DPF_OPS_PER_LEVEL
+    return ops
+
+
 def ragged_leg(groups_n: int, reps: int = 5):
     """Mixed DCF/DPF groups (1024 groups, log-uniform 2^6..2^18 elements, widths
     {8,16,24,32,64}, DCF:DPF 70:30): 1024 per-call launches on one stream vs one
@@ -165,6 +179,8 @@ def ragged_leg(groups_n: int, reps: int = 5):
         "batched_device_ms": batch_dev_ms,
         "plain_device_ms": plain_dev_ms,
         "batched_equals_plain": same,
+        "batched_valu_frac": ragged_ops(groups) / (batch_dev_ms * 1e-3) / 1e12 / VALU_PEAK_TOPS,
+        "plain_valu_frac": ragged_ops(groups) / (plain_dev_ms * 1e-3) / 1e12 / VALU_PEAK_TOPS,
         "groups_per_width": widths,
         "autotuned_tiles": {"%s/es%d" % ("dcf" if op == 0 else "dpf", es): t for (op, es), t in sorted(tiles.items())},
         "autotune_s": tune_s,
