@@ -230,6 +230,35 @@ def dpf_leg(log2n: int, reps: int = 10):
             "valu_frac": achieved / VALU_PEAK_TOPS}
 
 
+def spline_ops(w):
+    """Algorithmic VALU ops of one spline element: the range walks of the
+    interval sweep (one per interval end, plus one per interval start that does
+    not follow its predecessor), each n levels of (key schedule + 2 blocks) for
+    the seed and (key schedule + one block per 16-byte window of its slots) for
+    the value conversion, plus the final conversion."""
+    left = [int(v) for v in w.left.cpu().tolist()]
+    right = [int(v) for v in w.right.cpu().tolist()]
+    mask = (1 << w.n) - 1
+    C, OB, ks, blk = w.d + 1, (w.m + 7) // 8, 120, 320
+    g = w.I * C
+
+    def walk(j0, cnt):
+        if OB * g <= 16:
+            conv = 0
+        else:
+            nw = ((j0 + cnt) * OB - 1) // 16 - (j0 * OB) // 16 + 1
+            conv = ks + nw * blk
+        return w.n * (ks + 2 * blk + conv) + conv
+
+    ops = 0
+    for q in range(w.I):
+        if q == 0 or ((right[q - 1] + 1) & mask) != (left[q] & mask):
+            ops += walk(q * C, C)
+        link = q + 1 < w.I and ((right[q] + 1) & mask) == (left[q + 1] & mask)
+        ops += walk(q * C, 2 * C if link else C)
+    return ops
+
+
 def piecewise_leg(reps: int = 3):
     """DCF-MIC and a degree-2 spline, 16 intervals, bitWidthIn=24 / bitWidthOut=32, 2^22 elements;
     full-domain Grotto at bitWidthIn=16 over 4096 keys (2^28 points).  Party 0, keys resident."""
@@ -254,9 +283,11 @@ def piecewise_leg(reps: int = 3):
     out = torch.empty(N, dtype=torch.int32, device="cuda:0")
     W.spline_eval(w, 0, out, w.e, w.beta, cache, sp)
     wall, dev_ms = _timed(stream, lambda: W.spline_eval(w, 0, out, w.e, w.beta, cache, sp), reps)
+    ops = N * spline_ops(w)
     res["spline"] = {"workload": "DCF-spline degree 2, 16 contiguous intervals, bitWidthIn=24, bitWidthOut=32, "
                                  "2^22 elements, int32 coefficients",
-                     "elements_per_s": N / wall, "kernel_ms": dev_ms, "key_bytes": w.spl_ks}
+                     "elements_per_s": N / wall, "kernel_ms": dev_ms, "key_bytes": w.spl_ks,
+                     "ops_per_element": ops // N, "valu_frac": ops / (dev_ms * 1e-3) / 1e12 / VALU_PEAK_TOPS}
     del w, out, cache
     torch.cuda.empty_cache()
     gs = W.grotto_keys()

@@ -207,6 +207,67 @@ __device__ __forceinline__ void dcf_eval_range(const AesCtx &c, T acc[R], const 
     }
 }
 
+// dcf_eval_range for a compile-time output width OB = ceil(bitWidthOut/8):
+// the slots' conversion shares one key schedule per level across its AES
+// windows and extracts bytes at static positions (SlotConv).  Requires
+// cnt * OB <= 32 and cnt <= R.  Same results as dcf_eval_range.
+template <typename T, int OB, int R>
+__device__ __forceinline__ void dcf_eval_slots(const AesCtx &c, T acc[R], const uint8_t *sCW, const T *tail, T x,
+                                               const uint32_t seed[4], int party, int n, int g, int j0, int cnt)
+{
+    constexpr int NWORDS = (R * OB + 3) / 4;
+    SlotConv<OB, 3> cv;
+    cv.setup(g, j0, cnt);
+    uint32_t s[4] = {seed[0] & ~1u, seed[1], seed[2], seed[3]};
+    uint32_t t    = (uint32_t)party;
+    x             = mod_bits<T>(x, n);
+#pragma unroll
+    for (int r = 0; r < R; r++) acc[r] = 0;
+    const T tl = tail[(size_t)n * g], tr = tail[(size_t)n * g + 1];
+    for (int i = 0; i < n; i++)
+    {
+        const uint32_t xi = (uint32_t)(x >> (n - 1 - i)) & 1u;
+        uint32_t       cw[4];
+        load16(cw, sCW + 16 * i);
+        uint32_t blk[2][4];
+        blk[0][0] = 4u * xi, blk[0][1] = 0u, blk[0][2] = 4u * xi + 1u, blk[0][3] = 0u;
+        blk[1][0] = 4u * xi + 2u, blk[1][1] = 0u, blk[1][2] = 4u * xi + 3u, blk[1][3] = 0u;
+        aes_encrypt<2>(c, s, blk);
+        const uint32_t mk = 0u - t;
+        uint32_t       tn = blk[0][0] & 1u;
+        s[0]              = (blk[0][0] & ~1u) ^ (cw[0] & mk);
+        s[1]              = blk[0][1] ^ (cw[1] & mk);
+        s[2]              = blk[0][2] ^ (cw[2] & mk);
+        s[3]              = blk[0][3] ^ (cw[3] & mk);
+        tn ^= (uint32_t)(((xi ? tr : tl) >> i) & 1) & t;
+        uint32_t words[NWORDS];
+        cv.template fill<NWORDS>(c, blk[1], words);
+        const T *vcw = tail + (size_t)i * g + j0;
+#pragma unroll
+        for (int r = 0; r < R; r++)
+        {
+            if (r < cnt)
+            {
+                T term = (T)((T)SlotConv<OB, 3>::template at<NWORDS>(words, r) + (t ? vcw[r] : (T)0));
+                acc[r] = (T)(acc[r] + neg_if<T>(term, party));
+            }
+        }
+        t = tn;
+    }
+    uint32_t words[NWORDS];
+    cv.template fill<NWORDS>(c, s, words);
+    const T *last = tail + (size_t)n * g + 2 + j0;
+#pragma unroll
+    for (int r = 0; r < R; r++)
+    {
+        if (r < cnt)
+        {
+            T term = (T)((T)SlotConv<OB, 3>::template at<NWORDS>(words, r) + (t ? last[r] : (T)0));
+            acc[r] = (T)(acc[r] + neg_if<T>(term, party));
+        }
+    }
+}
+
 // Any groupSize: accumulate straight into the element's output slice.
 template <typename T>
 __device__ void dcf_eval_global(const AesCtx &c, T *out, const uint8_t *sCW, const T *tail, T x,

@@ -233,6 +233,31 @@ __device__ __forceinline__ void aes_encrypt(const AesCtx &c, const uint32_t key[
     for (int b = 0; b < NB; b++) aes_last(c, st[b], k);
 }
 
+// The first `nb` (<= NB, wave-uniform) of NB blocks under one key schedule;
+// blocks nb..NB-1 are left as they are.
+template <int NB>
+__device__ __forceinline__ void aes_encrypt_upto(const AesCtx &c, const uint32_t key[4], uint32_t st[NB][4], int nb)
+{
+    uint32_t k[4] = {key[0], key[1], key[2], key[3]};
+#pragma unroll
+    for (int b = 0; b < NB; b++)
+        if (b < nb) st[b][0] ^= k[0], st[b][1] ^= k[1], st[b][2] ^= k[2], st[b][3] ^= k[3];
+    const uint32_t rcon[10] = {0x01, 0x02, 0x04, 0x08, 0x10, 0x20, 0x40, 0x80, 0x1b, 0x36};
+#pragma unroll
+    for (int r = 0; r < 9; r++)
+    {
+        aes_next_key(c, k, rcon[r]);
+        uint32_t h[4] = {rot16(k[0]), rot16(k[1]), rot16(k[2]), rot16(k[3])};
+#pragma unroll
+        for (int b = 0; b < NB; b++)
+            if (b < nb) aes_round(c, st[b], h);
+    }
+    aes_next_key(c, k, rcon[9]);
+#pragma unroll
+    for (int b = 0; b < NB; b++)
+        if (b < nb) aes_last(c, st[b], k);
+}
+
 // E independent keys, NB blocks each, rounds interleaved across keys so the
 // lookups of all E*NB states are in flight together.
 template <int E, int NB>
@@ -356,6 +381,92 @@ struct ConvCtx
             if (i < 8) r |= (uint64_t)byte_of(buf, o & 15) << (8 * i);
         }
         return r;
+    }
+};
+
+// Value conversion restricted to the slots [j0, j0 + cnt) of a g-element
+// group with OB = ceil(bitWidthOut / 8) bytes per element (compile time); the
+// same bytes as ConvCtx::at (reference DcfConvertCtx, src/impl/dcf.h:69-137).
+// When OB * g > 16 the slots' bytes come from the AES windows w0 .. w0+nw-1
+// (window w = AES_value(0, 16w / OB)), all encrypted under one key schedule;
+// at most NW windows (the caller guarantees (j0*OB mod 16) + cnt*OB <= 16*NW).
+// fill() leaves the bytes starting at slot j0 in words[], so slot j0 + r is
+// bytes [r*OB, (r+1)*OB) of words[] -- static positions.
+template <int OB, int NW>
+struct SlotConv
+{
+    int direct, w0, nw, b0; // wave-uniform
+
+    __device__ __forceinline__ void setup(int g, int j0, int cnt)
+    {
+        const int start = j0 * OB, end = (j0 + cnt) * OB;
+        direct          = OB * g <= 16;
+        w0              = direct ? 0 : (start >> 4);
+        nw              = direct ? 1 : (((end - 1) >> 4) - w0 + 1);
+        b0              = start - 16 * w0;
+    }
+
+    template <int NWORDS>
+    __device__ __forceinline__ void fill(const AesCtx &c, const uint32_t v[4], uint32_t words[NWORDS]) const
+    {
+        static_assert(NWORDS + 4 <= 4 * (NW + 1), "window buffer too small");
+        uint32_t w[NW + 1][4];
+        if (direct)
+        {
+#pragma unroll
+            for (int i = 0; i < 4; i++) w[0][i] = v[i];
+#pragma unroll
+            for (int k = 1; k <= NW; k++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) w[k][i] = 0;
+        }
+        else
+        {
+            uint32_t blk[NW][4];
+#pragma unroll
+            for (int k = 0; k < NW; k++)
+                blk[k][0] = 0u, blk[k][1] = 0u, blk[k][2] = (uint32_t)((16 * (w0 + k)) / OB), blk[k][3] = 0u;
+            aes_encrypt_upto<NW>(c, v, blk, nw);
+#pragma unroll
+            for (int k = 0; k < NW; k++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) w[k][i] = blk[k][i];
+#pragma unroll
+            for (int i = 0; i < 4; i++) w[NW][i] = 0;
+        }
+        // shift the window bytes left by b0 (0..15): whole words by b0/4, then bytes by b0%4
+        const int      ws = b0 >> 2;
+        const uint32_t bs = (uint32_t)(b0 & 3);
+#pragma unroll
+        for (int q = 0; q < NWORDS; q++)
+        {
+            uint32_t lo = 0, hi = 0;
+#pragma unroll
+            for (int d = 0; d < 4; d++)
+            {
+                const int i = q + d;
+                if (ws == d)
+                {
+                    lo = w[i >> 2][i & 3];
+                    hi = w[(i + 1) >> 2][(i + 1) & 3];
+                }
+            }
+            words[q] = __builtin_amdgcn_alignbyte(hi, lo, bs);
+        }
+    }
+
+    // slot j0 + r from the filled words (r static)
+    template <int NWORDS>
+    __device__ __forceinline__ static uint64_t at(const uint32_t words[NWORDS], int r)
+    {
+        uint64_t v = 0;
+#pragma unroll
+        for (int i = 0; i < OB; i++)
+        {
+            const int o = r * OB + i;
+            v |= (uint64_t)((words[o >> 2] >> (8 * (o & 3))) & 0xffu) << (8 * i);
+        }
+        return v;
     }
 };
 

@@ -170,13 +170,26 @@ __global__ __launch_bounds__(kBlock) void spline_keygen_kernel(uint8_t *key, T *
 // Fast path: degree <= 7.  Each evaluation point x-1-L_0, x-1-(R_q+1) is walked
 // once; the walk at x-1-(R_q+1) also yields interval q+1's "current" slots when
 // intervals q and q+1 are adjacent (the reference copies the whole vector,
-// src/impl/spline.h:214-223).
-template <typename T>
+// src/impl/spline.h:214-223).  OB > 0 selects the slot conversion specialised
+// for ceil(bitWidthOut/8) = OB bytes (needs 2*(degree+1)*OB <= 32); OB = 0 the
+// generic one.
+template <typename T, int OB>
+__device__ __forceinline__ void spline_walk(const AesCtx &c, T *acc, const uint8_t *sc, const T *tail, T pt,
+                                            const uint32_t s[4], int party, int n, int m, int g, int j0, int cnt)
+{
+    if constexpr (OB == 0)
+        dcf_eval_range<T, 16>(c, acc, sc, tail, pt, s, party, n, m, g, j0, cnt);
+    else
+        dcf_eval_slots<T, OB, (32 / OB < 16 ? 32 / OB : 16)>(c, acc, sc, tail, pt, s, party, n, g, j0, cnt);
+}
+
+template <typename T, int OB>
 __global__ __launch_bounds__(kBlock) void spline_eval_kernel(T *out, const T *x, const uint8_t *key, const T *sE,
                                                              const T *sBeta, const uint8_t *seed, int party,
                                                              const T *left, const T *right, size_t I, int d, int n,
                                                              int m, size_t N)
 {
+    constexpr int RS = (OB == 0) ? 16 : (32 / OB < 16 ? 32 / OB : 16); // slots per walk
     FSS_KERNEL_PROLOGUE
     const int    C     = d + 1;
     const size_t g     = I * (size_t)C;
@@ -204,21 +217,21 @@ __global__ __launch_bounds__(kBlock) void spline_eval_kernel(T *out, const T *x,
             }
             else
             {
-                T acc[16];
-                dcf_eval_range<T, 16>(c, acc, sc, tail, (T)(xv - 1 - L), s, party, n, m, (int)g, (int)(q * C), C);
+                T acc[RS];
+                spline_walk<T, OB>(c, acc, sc, tail, (T)(xv - 1 - L), s, party, n, m, (int)g, (int)(q * C), C);
 #pragma unroll
-                for (int j = 0; j < 8; j++) cur[j] = acc[j];
+                for (int j = 0; j < 8; j++) cur[j] = (j < RS) ? acc[j < RS ? j : 0] : (T)0;
             }
             const bool link = (q + 1 < I) && mod_bits<T>((T)(right[q] + 1), n) == mod_bits<T>(left[q + 1], n);
-            T          acc[16];
-            dcf_eval_range<T, 16>(c, acc, sc, tail, (T)(xv - 1 - Rp), s, party, n, m, (int)g, (int)(q * C),
-                                  link ? 2 * C : C);
+            T          acc[RS];
+            spline_walk<T, OB>(c, acc, sc, tail, (T)(xv - 1 - Rp), s, party, n, m, (int)g, (int)(q * C),
+                               link ? 2 * C : C);
             const T cx = (T)((int)(xv > L) - (int)(xv > Rp));
 #pragma unroll
             for (int j = 0; j < 8; j++)
             {
                 if (j < C)
-                    w[j] = (T)(w[j] + (T)(cx * be[q * C + j]) - cur[j] + acc[j] + ee[q * C + j]);
+                    w[j] = (T)(w[j] + (T)(cx * be[q * C + j]) - cur[j] + acc[j < RS ? j : 0] + ee[q * C + j]);
             }
             // slots C..2C-1 of acc hold interval q+1's values at this point
 #pragma unroll
@@ -226,7 +239,7 @@ __global__ __launch_bounds__(kBlock) void spline_eval_kernel(T *out, const T *x,
             {
                 T v = 0;
 #pragma unroll
-                for (int r = 0; r < 16; r++) v = (r == j + C) ? acc[r] : v;
+                for (int r = 0; r < RS; r++) v = (r == j + C) ? acc[r] : v;
                 carry[j] = v;
             }
         }
@@ -313,14 +326,56 @@ int launch_mic_eval_rt(size_t es, int grid, hipStream_t st, void *out, const voi
     return launch_status();
 }
 
+template <typename T, int OB>
+static void spline_launch_ob(int grid, hipStream_t st, void *out, const void *x, const void *key, const void *e,
+                             const void *beta, const void *seed, int party, const void *left, const void *right,
+                             size_t I, int degree, int n, int m, size_t N)
+{
+    spline_eval_kernel<T, OB><<<grid, kBlock, 0, st>>>((T *)out, (const T *)x, (const uint8_t *)key, (const T *)e,
+                                                       (const T *)beta, (const uint8_t *)seed, party, (const T *)left,
+                                                       (const T *)right, I, degree, n, m, N);
+}
+
+// degree <= 7: the slot-conversion kernel for OB = ceil(m/8) when a linked
+// walk's 2(degree+1) slots fit 32 bytes, else the generic conversion.
+template <typename T>
+static void spline_launch(int grid, hipStream_t st, void *out, const void *x, const void *key, const void *e,
+                          const void *beta, const void *seed, int party, const void *left, const void *right, size_t I,
+                          int degree, int n, int m, size_t N)
+{
+    const int ob = (m + 7) / 8;
+    // (64-bit elements with 2-byte outputs keep the generic kernel: its register footprint is smaller)
+    const int fast = (2 * (degree + 1) * ob <= 32 && !(sizeof(T) == 8 && ob == 2)) ? ob : 0;
+#define FSS_SPLINE_OB(K)                                                                                            \
+    case K:                                                                                                         \
+        if constexpr (K <= (int)sizeof(T))                                                                          \
+        {                                                                                                           \
+            spline_launch_ob<T, K>(grid, st, out, x, key, e, beta, seed, party, left, right, I, degree, n, m, N);  \
+            return;                                                                                                 \
+        }                                                                                                           \
+        break;
+    switch (fast)
+    {
+        FSS_SPLINE_OB(1)
+        FSS_SPLINE_OB(2)
+        FSS_SPLINE_OB(3)
+        FSS_SPLINE_OB(4)
+        FSS_SPLINE_OB(5)
+        FSS_SPLINE_OB(6)
+        FSS_SPLINE_OB(7)
+        FSS_SPLINE_OB(8)
+        default: break;
+    }
+#undef FSS_SPLINE_OB
+    spline_launch_ob<T, 0>(grid, st, out, x, key, e, beta, seed, party, left, right, I, degree, n, m, N);
+}
+
 int launch_spline_eval_rt(size_t es, int grid, hipStream_t st, void *out, const void *x, const void *key,
                           const void *e, const void *beta, const void *seed, int party, const void *left,
                           const void *right, size_t I, int degree, int n, int m, size_t N)
 {
     if (degree > 7) return FAST_FSS_RUNTIME_ERROR;
-    FSS_DISPATCH(es, spline_eval_kernel<T><<<grid, kBlock, 0, st>>>(
-                         (T *)out, (const T *)x, (const uint8_t *)key, (const T *)e, (const T *)beta,
-                         (const uint8_t *)seed, party, (const T *)left, (const T *)right, I, degree, n, m, N));
+    FSS_DISPATCH(es, spline_launch<T>(grid, st, out, x, key, e, beta, seed, party, left, right, I, degree, n, m, N));
     return launch_status();
 }
 
@@ -450,12 +505,8 @@ extern "C" FAST_FSS_API int FastFss_cuda_dcfSplineEval(
     if (!aligned(key, 8) || !aligned(seed, 8)) return FAST_FSS_RUNTIME_ERROR;
     hipStream_t st = resolve_stream(cudaStreamPtr);
     if (degree <= 7)
-    {
-        FSS_DISPATCH(es, spline_eval_kernel<T><<<grid_for(N), kBlock, 0, st>>>(
-                             (T *)sharedOut, (const T *)maskedX, (const uint8_t *)key, (const T *)sharedE,
-                             (const T *)sharedBeta, (const uint8_t *)seed, partyId, (const T *)leftEndpoints,
-                             (const T *)rightEndpoints, I, (int)degree, (int)n, (int)m, N));
-    }
+        return launch_spline_eval_rt(es, grid_for(N), st, sharedOut, maskedX, key, sharedE, sharedBeta, seed, partyId,
+                                     leftEndpoints, rightEndpoints, I, (int)degree, (int)n, (int)m, N);
     else
     {
         FSS_DISPATCH(es, spline_eval_wide_kernel<T><<<grid_for(N), kBlock, 0, st>>>(

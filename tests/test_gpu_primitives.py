@@ -179,7 +179,9 @@ def _spline_case(rng, I, d, n, m, es):
 
 
 @pytest.mark.parametrize("n,m,es,I,d", [(8, 8, 1, 4, 2), (16, 16, 2, 5, 1), (24, 32, 4, 16, 2), (32, 32, 4, 3, 7),
-                                        (20, 20, 4, 2, 9), (48, 48, 8, 3, 2)])
+                                        (20, 20, 4, 2, 9), (48, 48, 8, 3, 2),
+                                        # 3-byte and 5-byte outputs: slots start at unaligned byte offsets
+                                        (24, 24, 4, 4, 2), (40, 40, 8, 3, 1), (12, 10, 2, 5, 4)])
 def test_spline(dev, n, m, es, I, d):
     rng = np.random.default_rng(n + I * d)
     N = 300
