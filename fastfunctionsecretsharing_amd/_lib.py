@@ -128,7 +128,7 @@ class ServeStats(ctypes.Structure):
     ]
 
 
-SERVE_DCF, SERVE_MIC, SERVE_SPLINE = 0, 1, 2
+SERVE_DCF, SERVE_MIC, SERVE_SPLINE, SERVE_DPF = 0, 1, 2, 3
 
 
 class BatchGroup(ctypes.Structure):

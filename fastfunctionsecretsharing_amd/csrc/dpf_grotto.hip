@@ -9,6 +9,7 @@
 #include "FastFss/hip/grotto_full.h"
 #include "dpf_device.h"
 #include "fss_host.h"
+#include "fss_launch.h"
 
 #include <atomic>
 
@@ -423,6 +424,15 @@ inline void grotto_full_split(int lv, size_t N, int &L, int &D)
     while (L < lv && (N << L) < target) L++;
     if (lv - L > kFullMaxDepth) L = lv - kFullMaxDepth;
     D = lv - L;
+}
+
+int launch_dpf_eval_g1_rt(size_t es, int grid, hipStream_t st, void *out, const void *x, const void *key,
+                          const void *seed, int party, int n, int m, size_t N)
+{
+    FSS_DISPATCH(es, dpf_eval_burst_kernel<T, 8><<<grid, kBlock, 0, st>>>((T *)out, (const T *)x,
+                                                                           (const uint8_t *)key,
+                                                                           (const uint8_t *)seed, party, n, m, N));
+    return launch_status();
 }
 
 } // namespace fss

@@ -13,6 +13,9 @@ namespace fss {
 int launch_dcf_eval_g1_rt(size_t es, int grid, hipStream_t st, void *out, const void *x, const void *key,
                           const void *seed, int party, int n, int m, size_t N);
 
+int launch_dpf_eval_g1_rt(size_t es, int grid, hipStream_t st, void *out, const void *x, const void *key,
+                          const void *seed, int party, int n, int m, size_t N);
+
 int launch_mic_eval_rt(size_t es, int grid, hipStream_t st, void *out, const void *x, const void *key, const void *z,
                        const void *seed, int party, const void *left, const void *right, size_t I, int n, int m,
                        size_t N);

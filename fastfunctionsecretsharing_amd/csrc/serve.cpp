@@ -73,6 +73,7 @@ static int validate(const FastFssServeRequest &r)
     switch (r.op)
     {
         case FAST_FSS_SERVE_DCF_EVAL: need = dcf_key_size(n, 1, es, N); break;
+        case FAST_FSS_SERVE_DPF_EVAL: need = dpf_key_size(n, 1, es, N); break;
         case FAST_FSS_SERVE_MIC_EVAL:
             if (r.intervalNum == 0 || !r.sharedZ || !r.leftEndpoints || !r.rightEndpoints)
                 return FAST_FSS_INVALID_BOUNDARY_DATA_SIZE_ERROR;
@@ -106,6 +107,9 @@ static int launch(Server *s, ServeItem *it, hipStream_t st)
     {
         case FAST_FSS_SERVE_DCF_EVAL:
             return launch_dcf_eval_g1_rt(r.elementSize, grid, st, r.sharedOut, r.maskedX, r.key, r.seed, r.partyId, n,
+                                         m, r.elementNum);
+        case FAST_FSS_SERVE_DPF_EVAL:
+            return launch_dpf_eval_g1_rt(r.elementSize, grid, st, r.sharedOut, r.maskedX, r.key, r.seed, r.partyId, n,
                                          m, r.elementNum);
         case FAST_FSS_SERVE_MIC_EVAL:
             return launch_mic_eval_rt(r.elementSize, grid, st, r.sharedOut, r.maskedX, r.key, r.sharedZ, r.seed,

@@ -206,11 +206,14 @@ def make_template(op, n, N, rng, gen, dev, sp):
     def ep(vals):
         return torch.tensor([v if v < 2**63 else v - 2**64 for v in vals], dtype=torch.int64).to(dt).to(dev)
 
-    if op == _lib.SERVE_DCF:
-        assert lib.FastFss_dcfGetKeyDataSize(ctypes.addressof(ks), n, n, 1, es, N) == 0
+    if op in (_lib.SERVE_DCF, _lib.SERVE_DPF):
+        dcf = op == _lib.SERVE_DCF
+        size_fn = lib.FastFss_dcfGetKeyDataSize if dcf else lib.FastFss_dpfGetKeyDataSize
+        assert size_fn(ctypes.addressof(ks), n, n, 1, es, N) == 0
         key = torch.empty(ks.value, dtype=torch.uint8, device=dev)
-        rc = lib.FastFss_cuda_dcfKeyGen(key.data_ptr(), ks.value, alpha.data_ptr(), es * N, None, 0,
-                                        seeds[0].data_ptr(), 16 * N, seeds[1].data_ptr(), 16 * N, n, n, 1, es, N, sp)
+        gen_fn = lib.FastFss_cuda_dcfKeyGen if dcf else lib.FastFss_cuda_dpfKeyGen
+        rc = gen_fn(key.data_ptr(), ks.value, alpha.data_ptr(), es * N, None, 0, seeds[0].data_ptr(), 16 * N,
+                    seeds[1].data_ptr(), 16 * N, n, n, 1, es, N, sp)
         assert rc == 0, rc
         return Template(op, n, es, N, masked, key, ks.value, seeds[0])
     I = MIC_INTERVALS if op == _lib.SERVE_MIC else SPLINE_INTERVALS
@@ -241,9 +244,10 @@ def make_template(op, n, N, rng, gen, dev, sp):
                     intervals=I, degree=d)
 
 
-def serving_templates(count=48, seed=5, lo=10, hi=20, widths=(16, 32, 64), device="cuda:0"):
-    """`count` resident key sets, op in {DCF, MIC, spline} round robin, size
-    log-uniform in [2^lo, 2^hi], bitWidthIn = bitWidthOut drawn from `widths`."""
+def serving_templates(count=48, seed=5, lo=10, hi=20, widths=(16, 32, 64), device="cuda:0",
+                      ops=(_lib.SERVE_DCF, _lib.SERVE_MIC, _lib.SERVE_SPLINE)):
+    """`count` resident key sets, op round robin over `ops`, size log-uniform in
+    [2^lo, 2^hi], bitWidthIn = bitWidthOut drawn from `widths`."""
     import torch
 
     dev = torch.device(device)
@@ -252,7 +256,7 @@ def serving_templates(count=48, seed=5, lo=10, hi=20, widths=(16, 32, 64), devic
     holder, sp = stream_ptr()
     out = []
     for i in range(count):
-        op = (_lib.SERVE_DCF, _lib.SERVE_MIC, _lib.SERVE_SPLINE)[i % 3]
+        op = ops[i % len(ops)]
         N = int(round(math.exp(rng.uniform(math.log(2**lo), math.log(2**hi)))))
         n = int(rng.choice(widths))
         out.append(make_template(op, n, N, rng, gen, dev, sp))
@@ -270,7 +274,7 @@ def serve_request(t, out):
     r.maskedX = t.masked.data_ptr()
     r.key, r.keyDataSize = t.key.data_ptr(), t.key_size
     r.seed = t.seed0.data_ptr()
-    if t.op != _lib.SERVE_DCF:
+    if t.op in (_lib.SERVE_MIC, _lib.SERVE_SPLINE):
         r.sharedZ = t.z.data_ptr()
         r.leftEndpoints, r.rightEndpoints = t.left.data_ptr(), t.right.data_ptr()
         r.intervalNum = t.intervals
@@ -285,9 +289,10 @@ def eval_template_alone(t, out, sp):
 
     lib = _lib.load()
     es, N, n = t.es, t.N, t.n
-    if t.op == _lib.SERVE_DCF:
-        rc = lib.FastFss_cuda_dcfEval(out.data_ptr(), es * N, t.masked.data_ptr(), es * N, t.key.data_ptr(),
-                                      t.key_size, t.seed0.data_ptr(), 16 * N, 0, n, n, 1, es, N, None, 0, sp)
+    if t.op in (_lib.SERVE_DCF, _lib.SERVE_DPF):
+        fn = lib.FastFss_cuda_dcfEval if t.op == _lib.SERVE_DCF else lib.FastFss_cuda_dpfEval
+        rc = fn(out.data_ptr(), es * N, t.masked.data_ptr(), es * N, t.key.data_ptr(), t.key_size,
+                t.seed0.data_ptr(), 16 * N, 0, n, n, 1, es, N, None, 0, sp)
     elif t.op == _lib.SERVE_MIC:
         I = t.intervals
         rc = lib.FastFss_cuda_dcfMICEval(out.data_ptr(), es * N * I, t.masked.data_ptr(), es * N, t.key.data_ptr(),

@@ -1,7 +1,7 @@
 /* Continuous-batching request server on MI355X (no counterpart in the
  * reference, whose C API is one synchronous-launch call per evaluation).
  *
- * Requests (DCF, DCF-MIC or DCF-spline evaluations whose keys and inputs are
+ * Requests (DCF, DPF, DCF-MIC or DCF-spline evaluations whose keys and inputs are
  * already resident in device memory) are submitted from any host thread.  A
  * scheduler thread admits queued requests in arrival order while
  *   - the number of requests in flight is below `maxInflight`, and
@@ -14,8 +14,8 @@
  * all in-flight requests across the CUs.
  *
  * Each request's output is bit-identical to the per-call entry point with
- * the same arguments (FastFss_cuda_dcfEval, FastFss_cuda_dcfMICEval,
- * FastFss_cuda_dcfSplineEval).  Latency = completion observed by the
+ * the same arguments (FastFss_cuda_dcfEval, FastFss_cuda_dpfEval,
+ * FastFss_cuda_dcfMICEval, FastFss_cuda_dcfSplineEval).  Latency = completion observed by the
  * scheduler minus submission.
  */
 #ifndef FAST_FSS_HIP_SERVE_H
@@ -34,6 +34,7 @@ enum FastFssServeOp
     FAST_FSS_SERVE_DCF_EVAL    = 0, /* groupSize 1 */
     FAST_FSS_SERVE_MIC_EVAL    = 1,
     FAST_FSS_SERVE_SPLINE_EVAL = 2, /* degree <= 7 */
+    FAST_FSS_SERVE_DPF_EVAL    = 3, /* groupSize 1 */
 };
 
 typedef struct FastFssServeRequest

@@ -48,6 +48,19 @@ def test_mix_equals_alone(env, budget, inflight, streams, mode):
         assert torch.equal(a, b), (i, templates[picks[i]].op, templates[picks[i]].n)
 
 
+def test_mix_with_dpf_equals_alone(env):
+    torch, _lib, W, _ = env
+    templates = W.serving_templates(count=8, seed=21, lo=6, hi=12,
+                                    ops=(_lib.SERVE_DPF, _lib.SERVE_DCF, _lib.SERVE_MIC, _lib.SERVE_SPLINE))
+    rng = np.random.default_rng(3)
+    picks = [int(p) for p in rng.integers(0, len(templates), size=24)]
+    outs, lats, wall, st = W.run_trace(templates, picks, np.zeros(len(picks)), 1 << 34, 8, 4, 1)
+    assert st.completed == len(picks) and st.failed == 0
+    ref = _alone(torch, W, templates, picks)
+    for i, (a, b) in enumerate(zip(outs, ref)):
+        assert torch.equal(a, b), (i, templates[picks[i]].op)
+
+
 def test_submit_validation(env):
     import ctypes
 
