@@ -73,6 +73,32 @@ __global__ __launch_bounds__(kBlock) void dpf_eval_burst_kernel(T *out, const T 
     }
 }
 
+// Work-queue form (see dcf_eval_queue_kernel): 64-element tiles per wave.
+template <typename T, int CH>
+__global__ __launch_bounds__(kBlock) void dpf_eval_queue_kernel(T *out, const T *x, const uint8_t *key,
+                                                                const uint8_t *seed, int party, int n, int m, size_t N,
+                                                                uint32_t *counter)
+{
+    FSS_KERNEL_PROLOGUE
+    const T       *tails = (const T *)(key + N * 16 * (size_t)n);
+    const uint32_t lane  = threadIdx.x & 63u;
+    const uint32_t tiles = (uint32_t)((N + 63) / 64);
+    for (;;)
+    {
+        uint32_t tile = 0;
+        if (lane == 0) tile = atomicAdd(counter, 1u);
+        tile = __builtin_amdgcn_readfirstlane(tile);
+        if (tile >= tiles) break;
+        const size_t e = (size_t)tile * 64 + lane;
+        if (e < N)
+        {
+            uint32_t s[4];
+            load16(s, seed + 16 * e);
+            out[e] = dpf_eval_g1_burst<T, CH>(c, key + e * 16 * (size_t)n, tails + e * 3, x[e], s, party, n, m);
+        }
+    }
+}
+
 // One work item per (element, point): point j of element e lands at
 // out[e][(x_e - j) mod 2^n] (src/kernel/dpf.h:253-276).
 template <typename T>
@@ -502,7 +528,16 @@ extern "C" FAST_FSS_API int FastFss_cuda_dpfEval(void *sharedOut, size_t sharedO
     if (N == 0) return FAST_FSS_SUCCESS;
     if (!aligned(key, 8) || !aligned(seed, 8)) return FAST_FSS_RUNTIME_ERROR;
     hipStream_t st = resolve_stream(cudaStreamPtr);
-    if (g == 1)
+    if (g == 1 && N >= ((size_t)1 << 16))
+    {
+        uint32_t *counter = queue_counter_acquire(st);
+        if (!counter) return FAST_FSS_RUNTIME_ERROR;
+        FSS_DISPATCH(es, dpf_eval_queue_kernel<T, 8><<<grid_for(N), kBlock, 0, st>>>(
+                             (T *)sharedOut, (const T *)maskedX, (const uint8_t *)key, (const uint8_t *)seed,
+                             partyId, (int)n, (int)m, N, counter));
+        queue_counter_release(st, counter);
+    }
+    else if (g == 1)
     {
         FSS_DISPATCH(es, dpf_eval_burst_kernel<T, 8><<<grid_for(N), kBlock, 0, st>>>(
                              (T *)sharedOut, (const T *)maskedX, (const uint8_t *)key, (const uint8_t *)seed,

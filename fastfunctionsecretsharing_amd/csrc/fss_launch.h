@@ -7,8 +7,14 @@
 
 #include <hip/hip_runtime.h>
 #include <stddef.h>
+#include <stdint.h>
 
 namespace fss {
+
+// Zeroed 32-bit work-queue counter in stream-ordered memory on `st`
+// (nullptr on failure); release it on the same stream after the launch.
+uint32_t *queue_counter_acquire(hipStream_t st);
+void      queue_counter_release(hipStream_t st, uint32_t *p);
 
 int launch_dcf_eval_g1_rt(size_t es, int grid, hipStream_t st, void *out, const void *x, const void *key,
                           const void *seed, int party, int n, int m, size_t N);

@@ -97,6 +97,27 @@ def _sub_key(key, idx, n, g, es, N):
     return np.concatenate([sc.reshape(-1), tl.reshape(-1)])
 
 
+@pytest.mark.parametrize("es,n", [(1, 8), (2, 13), (4, 32), (8, 64)])
+@pytest.mark.parametrize("N", [5000, 70001])
+def test_eval_variants_agree(dev, es, n, N):
+    """Every groupSize-1 launch variant (static bursts, elements per lane, the
+    work queue) gives the oracle's shares; 70001 elements end in a partial tile."""
+    rng = np.random.default_rng(es * 1000 + N)
+    alpha, beta, masked, s0, s1 = _case(rng, N, n, n, 1, es)
+    key_o = dcf_keygen(OR, alpha, beta, s0, s1, n, n, 1, es)
+    key_d = dev.put(key_o)
+    want = dcf_eval(OR, masked, key_o, s1, 1, n, n, 1, es)
+    lib = dev.lib
+    try:
+        for v in range(lib.FastFss_hip_getEvalVariantCount()):
+            assert lib.FastFss_hip_setEvalVariant(v) == 0
+            got = dev.get(dcf_eval(dev, masked, key_d, s1, 1, n, n, 1, es))
+            assert np.array_equal(got, want), v
+    finally:
+        lib.FastFss_hip_setEvalVariant(-1)
+    assert lib.FastFss_hip_setEvalVariant(lib.FastFss_hip_getEvalVariantCount()) == -1
+
+
 def test_dcf_2pow24_n32(dev):
     """Plain DCF, bitWidthIn = bitWidthOut = 32, 2^24 elements, one key each."""
     import torch

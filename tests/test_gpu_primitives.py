@@ -73,6 +73,25 @@ def test_dpf_eval_all_and_multi(dev, n, es, g):
                               dpf_eval_multi(OR, x, ko, sd, party, pts, n, m, g, es))
 
 
+@pytest.mark.parametrize("es,n,m", [(2, 16, 8), (4, 32, 32), (8, 64, 20)])
+def test_dpf_queue_size(dev, es, n, m):
+    """From 2^16 elements the groupSize-1 DPF eval runs on the work queue; 70001
+    elements end in a partial tile."""
+    rng = np.random.default_rng(es + n)
+    N = 70001
+    alpha = rand_elems(rng, N, n, es)
+    x = rand_elems(rng, N, n, es)
+    x[::5] = alpha[::5]
+    beta = rand_elems(rng, N, m, es)
+    s0, s1 = rand_seeds(rng, N), rand_seeds(rng, N)
+    ko = dpf_keygen(OR, alpha, beta, s0, s1, n, m, 1, es)
+    kd = dpf_keygen(dev, alpha, beta, s0, s1, n, m, 1, es)
+    assert np.array_equal(dev.get(kd), ko)
+    for party, sd in ((0, s0), (1, s1)):
+        assert np.array_equal(dev.get(dpf_eval(dev, x, kd, sd, party, n, m, 1, es)),
+                              dpf_eval(OR, x, ko, sd, party, n, m, 1, es)), party
+
+
 # --------------------------------------------------------------- Grotto ----
 
 
