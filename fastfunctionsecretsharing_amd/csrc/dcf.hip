@@ -15,7 +15,7 @@ namespace fss {
 template <typename T>
 __global__ __launch_bounds__(kBlock) void dcf_keygen_g1_kernel(uint8_t *key, const T *alpha, const T *beta,
                                                                const uint8_t *seed0, const uint8_t *seed1, int n,
-                                                               int m, size_t N)
+                                                               int m, size_t N, uint32_t *counter)
 {
     __shared__ uint32_t lds[kLdsTableWords];
     aes_fill_lds(lds);
@@ -23,14 +23,13 @@ __global__ __launch_bounds__(kBlock) void dcf_keygen_g1_kernel(uint8_t *key, con
     const AesCtx c      = aes_ctx(lds);
     const size_t stride = (size_t)n + 3;
     uint8_t     *tails  = key + N * 16 * (size_t)n;
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < N; e += (size_t)gridDim.x * blockDim.x)
-    {
+    for_each_item(N, counter, [&](size_t e) {
         uint32_t a[4], b[4];
         load16(a, seed0 + 16 * e);
         load16(b, seed1 + 16 * e);
         dcf_keygen_g1<T, 4>(c, key + e * 16 * (size_t)n, (T *)tails + e * stride, alpha[e], beta ? beta[e] : (T)1,
                             a, b, n, m);
-    }
+    });
 }
 
 template <typename T>
@@ -307,9 +306,12 @@ extern "C" FAST_FSS_API int FastFss_cuda_dcfKeyGen(void *key, size_t keyDataSize
     const void *b  = betaDataSize ? beta : nullptr;
     if (g == 1)
     {
+        uint32_t *counter = queue_counter_acquire(st);
+        if (!counter) return FAST_FSS_RUNTIME_ERROR;
         FSS_DISPATCH(es, dcf_keygen_g1_kernel<T><<<grid_for(N), kBlock, 0, st>>>(
                              (uint8_t *)key, (const T *)alpha, (const T *)b, (const uint8_t *)seed0,
-                             (const uint8_t *)seed1, (int)n, (int)m, N));
+                             (const uint8_t *)seed1, (int)n, (int)m, N, counter));
+        queue_counter_release(st, counter);
     }
     else
     {

@@ -30,18 +30,17 @@ namespace fss {
 template <typename T>
 __global__ __launch_bounds__(kBlock) void dpf_keygen_kernel(uint8_t *key, const T *alpha, const T *beta,
                                                             const uint8_t *seed0, const uint8_t *seed1, int n, int m,
-                                                            int g, size_t N)
+                                                            int g, size_t N, uint32_t *counter)
 {
     FSS_KERNEL_PROLOGUE
     T *tails = (T *)(key + N * 16 * (size_t)n);
-    FSS_GRID_LOOP(e, N)
-    {
+    for_each_item(N, counter, [&](size_t e) {
         uint32_t a[4], b[4];
         load16(a, seed0 + 16 * e);
         load16(b, seed1 + 16 * e);
         dpf_keygen_one<T>(c, key + e * 16 * (size_t)n, tails + e * (size_t)(2 + g), alpha[e],
                           beta ? beta + e * (size_t)g : nullptr, a, b, n, m, g);
-    }
+    });
 }
 
 template <typename T>
@@ -492,10 +491,13 @@ extern "C" FAST_FSS_API int FastFss_cuda_dpfKeyGen(void *key, size_t keyDataSize
     if (!aligned(key, 8) || !aligned(seed0, 8) || !aligned(seed1, 8)) return FAST_FSS_RUNTIME_ERROR;
     hipStream_t st = resolve_stream(cudaStreamPtr);
     const void *b  = betaDataSize ? beta : nullptr;
+    uint32_t *counter = queue_counter_acquire(st);
+    if (!counter) return FAST_FSS_RUNTIME_ERROR;
     FSS_DISPATCH(es, dpf_keygen_kernel<T><<<grid_for(N), kBlock, 0, st>>>((uint8_t *)key, (const T *)alpha,
                                                                            (const T *)b, (const uint8_t *)seed0,
                                                                            (const uint8_t *)seed1, (int)n, (int)m,
-                                                                           (int)g, N));
+                                                                           (int)g, N, counter));
+    queue_counter_release(st, counter);
     return launch_status();
 }
 

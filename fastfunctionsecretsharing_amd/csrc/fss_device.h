@@ -471,6 +471,44 @@ struct SlotConv
 };
 
 // ---------------------------------------------------------------------------
+// work distribution
+// ---------------------------------------------------------------------------
+
+// body(i) for every work item i in [0, count), in 64-item tiles: wave w takes
+// tile t and lane L item 64*t + L.  With a queue counter (zeroed before the
+// launch) each wave takes the next tile from it until none are left; without
+// one, wave w takes tiles w, w + W, w + 2W, ... (W waves in the grid).  One
+// loop body serves both, so the kernel is compiled once.  Every lane of a
+// wave reaches the loop head together, so the tile read by lane 0 is the
+// wave's.
+template <typename Body>
+__device__ __forceinline__ void for_each_item(size_t count, uint32_t *counter, Body &&body)
+{
+    const uint32_t lane   = threadIdx.x & 63u;
+    const size_t   tiles  = (count + 63) / 64;
+    const size_t   waves  = ((size_t)gridDim.x * blockDim.x) >> 6;
+    size_t         stat   = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    for (;;)
+    {
+        size_t tile;
+        if (counter)
+        {
+            uint32_t t = 0;
+            if (lane == 0) t = atomicAdd(counter, 1u);
+            tile = __builtin_amdgcn_readfirstlane(t);
+        }
+        else
+        {
+            tile = stat;
+            stat += waves;
+        }
+        if (tile >= tiles) break;
+        const size_t i = tile * 64 + lane;
+        if (i < count) body(i);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // memory helpers
 // ---------------------------------------------------------------------------
 

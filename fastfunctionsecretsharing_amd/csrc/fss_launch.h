@@ -22,13 +22,14 @@ int launch_dcf_eval_g1_rt(size_t es, int grid, hipStream_t st, void *out, const 
 int launch_dpf_eval_g1_rt(size_t es, int grid, hipStream_t st, void *out, const void *x, const void *key,
                           const void *seed, int party, int n, int m, size_t N);
 
+// counter: zeroed work-queue counter, or nullptr for the static grid split
 int launch_mic_eval_rt(size_t es, int grid, hipStream_t st, void *out, const void *x, const void *key, const void *z,
                        const void *seed, int party, const void *left, const void *right, size_t I, int n, int m,
-                       size_t N);
+                       size_t N, uint32_t *counter);
 
 // degree <= 7 (the path that needs no per-element workspace)
 int launch_spline_eval_rt(size_t es, int grid, hipStream_t st, void *out, const void *x, const void *key,
                           const void *e, const void *beta, const void *seed, int party, const void *left,
-                          const void *right, size_t I, int degree, int n, int m, size_t N);
+                          const void *right, size_t I, int degree, int n, int m, size_t N, uint32_t *counter);
 
 } // namespace fss

@@ -30,13 +30,12 @@ namespace fss {
 template <typename T>
 __global__ __launch_bounds__(kBlock) void mic_keygen_kernel(uint8_t *key, T *z, const T *alpha, const uint8_t *seed0,
                                                             const uint8_t *seed1, const T *left, const T *right,
-                                                            size_t I, int n, int m, size_t N)
+                                                            size_t I, int n, int m, size_t N, uint32_t *counter)
 {
     FSS_KERNEL_PROLOGUE
     T *tails = (T *)(key + N * 16 * (size_t)n);
     const T MAX = mod_bits<T>((T)~(T)0, n);
-    FSS_GRID_LOOP(e, N)
-    {
+    for_each_item(N, counter, [&](size_t e) {
         uint32_t a[4], b[4];
         load16(a, seed0 + 16 * e);
         load16(b, seed1 + 16 * e);
@@ -53,7 +52,7 @@ __global__ __launch_bounds__(kBlock) void mic_keygen_kernel(uint8_t *key, T *z, 
             const int zv = (int)(aP > aQ) - (int)(aP > p) + (int)(aQp > qP) + (int)(aQ == MAX);
             z[e * I + i] = (T)zv;
         }
-    }
+    });
 }
 
 // Interval shares [first, first+cnt) of one element (src/impl/mic.h:43-109).
@@ -101,22 +100,21 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void mic_eval_kernel(T *out, const T *x, const uint8_t *key, const T *z,
                                                           const uint8_t *seed, int party, const T *left,
                                                           const T *right, size_t I, size_t blocksPerElem, int n,
-                                                          int outBytes, size_t N)
+                                                          int outBytes, size_t N, uint32_t *counter)
 {
     FSS_KERNEL_PROLOGUE
     const T     *tails = (const T *)(key + N * 16 * (size_t)n);
     const size_t span  = (I + blocksPerElem - 1) / blocksPerElem;
-    FSS_GRID_LOOP(w, N * blocksPerElem)
-    {
+    for_each_item(N * blocksPerElem, counter, [&](size_t w) {
         const size_t e = w / blocksPerElem, blk = w % blocksPerElem;
         const size_t first = blk * span;
-        if (first >= I) continue;
+        if (first >= I) return;
         const size_t cnt = (first + span < I) ? span : (I - first);
         uint32_t     s[4];
         load16(s, seed + 16 * e);
         mic_eval_range<T>(c, out + e * I, z + e * I, key + e * 16 * (size_t)n, tails + e * (size_t)(n + 3), x[e], s,
                           party, left, right, first, cnt, n, outBytes);
-    }
+    });
 }
 
 // --------------------------------------------------------------- spline ----
@@ -187,15 +185,14 @@ template <typename T, int OB>
 __global__ __launch_bounds__(kBlock) void spline_eval_kernel(T *out, const T *x, const uint8_t *key, const T *sE,
                                                              const T *sBeta, const uint8_t *seed, int party,
                                                              const T *left, const T *right, size_t I, int d, int n,
-                                                             int m, size_t N)
+                                                             int m, size_t N, uint32_t *counter)
 {
     constexpr int RS = (OB == 0) ? 16 : (32 / OB < 16 ? 32 / OB : 16); // slots per walk
     FSS_KERNEL_PROLOGUE
     const int    C     = d + 1;
     const size_t g     = I * (size_t)C;
     const T     *tails = (const T *)(key + N * 16 * (size_t)n);
-    FSS_GRID_LOOP(e, N)
-    {
+    for_each_item(N, counter, [&](size_t e) {
         uint32_t s[4];
         load16(s, seed + 16 * e);
         const uint8_t *sc   = key + e * 16 * (size_t)n;
@@ -254,7 +251,7 @@ __global__ __launch_bounds__(kBlock) void spline_eval_kernel(T *out, const T *x,
             }
         }
         out[e] = mod_bits<T>(r, m);
-    }
+    });
 }
 
 // Any degree: the reference's per-element workspace inside the caller's
@@ -316,24 +313,24 @@ static size_t mic_blocks_per_elem(size_t N, size_t I)
 
 int launch_mic_eval_rt(size_t es, int grid, hipStream_t st, void *out, const void *x, const void *key, const void *z,
                        const void *seed, int party, const void *left, const void *right, size_t I, int n, int m,
-                       size_t N)
+                       size_t N, uint32_t *counter)
 {
     const size_t bpe = mic_blocks_per_elem(N, I);
     FSS_DISPATCH(es, mic_eval_kernel<T><<<grid, kBlock, 0, st>>>((T *)out, (const T *)x, (const uint8_t *)key,
                                                                  (const T *)z, (const uint8_t *)seed, party,
                                                                  (const T *)left, (const T *)right, I, bpe, n,
-                                                                 (m + 7) / 8, N));
+                                                                 (m + 7) / 8, N, counter));
     return launch_status();
 }
 
 template <typename T, int OB>
 static void spline_launch_ob(int grid, hipStream_t st, void *out, const void *x, const void *key, const void *e,
                              const void *beta, const void *seed, int party, const void *left, const void *right,
-                             size_t I, int degree, int n, int m, size_t N)
+                             size_t I, int degree, int n, int m, size_t N, uint32_t *counter)
 {
     spline_eval_kernel<T, OB><<<grid, kBlock, 0, st>>>((T *)out, (const T *)x, (const uint8_t *)key, (const T *)e,
                                                        (const T *)beta, (const uint8_t *)seed, party, (const T *)left,
-                                                       (const T *)right, I, degree, n, m, N);
+                                                       (const T *)right, I, degree, n, m, N, counter);
 }
 
 // degree <= 7: the slot-conversion kernel for OB = ceil(m/8) when a linked
@@ -341,7 +338,7 @@ static void spline_launch_ob(int grid, hipStream_t st, void *out, const void *x,
 template <typename T>
 static void spline_launch(int grid, hipStream_t st, void *out, const void *x, const void *key, const void *e,
                           const void *beta, const void *seed, int party, const void *left, const void *right, size_t I,
-                          int degree, int n, int m, size_t N)
+                          int degree, int n, int m, size_t N, uint32_t *counter)
 {
     const int ob = (m + 7) / 8;
     // (64-bit elements with 2-byte outputs keep the generic kernel: its register footprint is smaller)
@@ -350,7 +347,8 @@ static void spline_launch(int grid, hipStream_t st, void *out, const void *x, co
     case K:                                                                                                         \
         if constexpr (K <= (int)sizeof(T))                                                                          \
         {                                                                                                           \
-            spline_launch_ob<T, K>(grid, st, out, x, key, e, beta, seed, party, left, right, I, degree, n, m, N);  \
+            spline_launch_ob<T, K>(grid, st, out, x, key, e, beta, seed, party, left, right, I, degree, n, m, N,   \
+                                   counter);                                                                        \
             return;                                                                                                 \
         }                                                                                                           \
         break;
@@ -367,15 +365,16 @@ static void spline_launch(int grid, hipStream_t st, void *out, const void *x, co
         default: break;
     }
 #undef FSS_SPLINE_OB
-    spline_launch_ob<T, 0>(grid, st, out, x, key, e, beta, seed, party, left, right, I, degree, n, m, N);
+    spline_launch_ob<T, 0>(grid, st, out, x, key, e, beta, seed, party, left, right, I, degree, n, m, N, counter);
 }
 
 int launch_spline_eval_rt(size_t es, int grid, hipStream_t st, void *out, const void *x, const void *key,
                           const void *e, const void *beta, const void *seed, int party, const void *left,
-                          const void *right, size_t I, int degree, int n, int m, size_t N)
+                          const void *right, size_t I, int degree, int n, int m, size_t N, uint32_t *counter)
 {
     if (degree > 7) return FAST_FSS_RUNTIME_ERROR;
-    FSS_DISPATCH(es, spline_launch<T>(grid, st, out, x, key, e, beta, seed, party, left, right, I, degree, n, m, N));
+    FSS_DISPATCH(es, spline_launch<T>(grid, st, out, x, key, e, beta, seed, party, left, right, I, degree, n, m, N,
+                                      counter));
     return launch_status();
 }
 
@@ -405,9 +404,12 @@ extern "C" FAST_FSS_API int FastFss_cuda_dcfMICKeyGen(void *key, size_t keyDataS
     if (N == 0) return FAST_FSS_SUCCESS;
     if (!aligned(key, 8) || !aligned(seed0, 8) || !aligned(seed1, 8)) return FAST_FSS_RUNTIME_ERROR;
     hipStream_t st = resolve_stream(cudaStreamPtr);
+    uint32_t *counter = queue_counter_acquire(st);
+    if (!counter) return FAST_FSS_RUNTIME_ERROR;
     FSS_DISPATCH(es, mic_keygen_kernel<T><<<grid_for(N), kBlock, 0, st>>>(
                          (uint8_t *)key, (T *)z, (const T *)alpha, (const uint8_t *)seed0, (const uint8_t *)seed1,
-                         (const T *)leftEndpoints, (const T *)rightEndpoints, I, (int)n, (int)m, N));
+                         (const T *)leftEndpoints, (const T *)rightEndpoints, I, (int)n, (int)m, N, counter));
+    queue_counter_release(st, counter);
     return launch_status();
 }
 
@@ -438,12 +440,13 @@ extern "C" FAST_FSS_API int FastFss_cuda_dcfMICEval(void *sharedOut, size_t shar
     if (N == 0 || I == 0) return FAST_FSS_SUCCESS;
     if (!aligned(key, 8) || !aligned(seed, 8)) return FAST_FSS_RUNTIME_ERROR;
     hipStream_t st = resolve_stream(cudaStreamPtr);
-    const size_t bpe = mic_blocks_per_elem(N, I);
-    FSS_DISPATCH(es, mic_eval_kernel<T><<<grid_for(N * bpe), kBlock, 0, st>>>(
-                         (T *)sharedOut, (const T *)maskedX, (const uint8_t *)key, (const T *)sharedZ,
-                         (const uint8_t *)seed, partyId, (const T *)leftEndpoints, (const T *)rightEndpoints, I, bpe,
-                         (int)n, (int)((m + 7) / 8), N));
-    return launch_status();
+    const size_t bpe     = mic_blocks_per_elem(N, I);
+    uint32_t    *counter = queue_counter_acquire(st);
+    if (!counter) return FAST_FSS_RUNTIME_ERROR;
+    const int rc = launch_mic_eval_rt(es, grid_for(N * bpe), st, sharedOut, maskedX, key, sharedZ, seed, partyId,
+                                      leftEndpoints, rightEndpoints, I, (int)n, (int)m, N, counter);
+    queue_counter_release(st, counter);
+    return rc;
 }
 
 extern "C" FAST_FSS_API int FastFss_cuda_dcfSplineKeyGen(
@@ -505,8 +508,15 @@ extern "C" FAST_FSS_API int FastFss_cuda_dcfSplineEval(
     if (!aligned(key, 8) || !aligned(seed, 8)) return FAST_FSS_RUNTIME_ERROR;
     hipStream_t st = resolve_stream(cudaStreamPtr);
     if (degree <= 7)
-        return launch_spline_eval_rt(es, grid_for(N), st, sharedOut, maskedX, key, sharedE, sharedBeta, seed, partyId,
-                                     leftEndpoints, rightEndpoints, I, (int)degree, (int)n, (int)m, N);
+    {
+        uint32_t *counter = queue_counter_acquire(st);
+        if (!counter) return FAST_FSS_RUNTIME_ERROR;
+        const int rc = launch_spline_eval_rt(es, grid_for(N), st, sharedOut, maskedX, key, sharedE, sharedBeta, seed,
+                                             partyId, leftEndpoints, rightEndpoints, I, (int)degree, (int)n, (int)m,
+                                             N, counter);
+        queue_counter_release(st, counter);
+        return rc;
+    }
     else
     {
         FSS_DISPATCH(es, spline_eval_wide_kernel<T><<<grid_for(N), kBlock, 0, st>>>(

@@ -114,11 +114,11 @@ static int launch(Server *s, ServeItem *it, hipStream_t st)
         case FAST_FSS_SERVE_MIC_EVAL:
             return launch_mic_eval_rt(r.elementSize, grid, st, r.sharedOut, r.maskedX, r.key, r.sharedZ, r.seed,
                                       r.partyId, r.leftEndpoints, r.rightEndpoints, r.intervalNum, n, m,
-                                      r.elementNum);
+                                      r.elementNum, nullptr);
         default:
             return launch_spline_eval_rt(r.elementSize, grid, st, r.sharedOut, r.maskedX, r.key, r.sharedZ,
                                          r.sharedBeta, r.seed, r.partyId, r.leftEndpoints, r.rightEndpoints,
-                                         r.intervalNum, (int)r.degree, n, m, r.elementNum);
+                                         r.intervalNum, (int)r.degree, n, m, r.elementNum, nullptr);
     }
 }
 
