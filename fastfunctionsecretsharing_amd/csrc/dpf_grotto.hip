@@ -98,27 +98,6 @@ __global__ __launch_bounds__(kBlock) void dpf_eval_queue_kernel(T *out, const T 
     }
 }
 
-// One work item per (element, point): point j of element e lands at
-// out[e][(x_e - j) mod 2^n] (src/kernel/dpf.h:253-276).
-template <typename T>
-__global__ __launch_bounds__(kBlock) void dpf_eval_all_kernel(T *out, const T *x, const uint8_t *key,
-                                                              const uint8_t *seed, int party, int n, int m, int g,
-                                                              size_t N)
-{
-    FSS_KERNEL_PROLOGUE
-    const T     *tails = (const T *)(key + N * 16 * (size_t)n);
-    const size_t size  = (size_t)1 << n;
-    FSS_GRID_LOOP(w, N * size)
-    {
-        const size_t e = w >> n, j = w & (size - 1);
-        uint32_t     s[4];
-        load16(s, seed + 16 * e);
-        const size_t idx = ((size_t)x[e] - j) & (size - 1);
-        dpf_eval_one<T>(c, out + (e * size + idx) * g, key + e * 16 * (size_t)n, tails + e * (size_t)(2 + g), (T)j,
-                        s, party, n, m, g);
-    }
-}
-
 // One work item per (element, point): out[e][j] = Eval(x_e - point_j)
 // (src/kernel/dpf.h:481-534).
 template <typename T>
@@ -437,11 +416,123 @@ __global__ __launch_bounds__(kBlock) void grotto_full_kernel(void *out, const ui
     }
 }
 
-// Subtree split: enough lanes to fill every CU once (or the configured
-// target), and at most kFullMaxDepth levels expanded per lane.
+// DPF evalAll (src/kernel/dpf.h:182-277): element e's share at point j goes
+// to out[e][(x_e - j) mod 2^n].  Lane (e, r) walks L levels to subtree r and
+// expands its D = n - L levels depth-first like grotto_full_kernel (one key
+// schedule and two blocks per inner node, pending right children in a
+// register stack), converting each leaf seed as dpf_eval_one does, so every
+// output equals dpf_eval_one at that point.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void dpf_eval_all_dfs_kernel(T *out, const T *x, const uint8_t *key,
+                                                                  const uint8_t *seed, int party, int n, int m, int g,
+                                                                  int L, int D, size_t N)
+{
+    FSS_KERNEL_PROLOGUE
+    const T     *tails = (const T *)(key + N * 16 * (size_t)n);
+    const size_t lanes = N << L;
+    const size_t size  = (size_t)1 << n;
+    FSS_GRID_LOOP(w, lanes)
+    {
+        const size_t   e  = w >> L;
+        const uint64_t r  = (uint64_t)w & ((1ull << L) - 1ull);
+        const uint8_t *sc = key + e * 16 * (size_t)n;
+        const T       *tl = tails + e * (size_t)(2 + g);
+        const T        tL = tl[0], tR = tl[1];
+        const size_t   xe = (size_t)mod_bits<T>(x[e], n);
+        T             *oe = out + e * size * (size_t)g;
+        uint32_t       s[4];
+        load16(s, seed + 16 * e);
+        s[0] &= ~1u;
+        uint32_t t = (uint32_t)party;
+        for (int i = 0; i < L; i++)
+            tree_eval_step<T>(c, s, t, (uint32_t)(r >> (L - 1 - i)) & 1u, sc + 16 * i, tL, tR, i);
+
+        uint32_t       stS[kFullMaxDepth][4];
+        uint32_t       stT[kFullMaxDepth];
+        const uint32_t leaves = 1u << D;
+        int            depth  = 0;
+        for (uint32_t leaf = 0;;)
+        {
+            while (depth < D)
+            {
+                const int level     = L + depth;
+                uint32_t  blk[2][4] = {{0u, 0u, 1u, 0u}, {2u, 0u, 3u, 0u}};
+                uint32_t  cw[4];
+                load16(cw, sc + 16 * level);
+                aes_encrypt<2>(c, s, blk);
+                const uint32_t mk = 0u - t;
+                const uint32_t tl_ = (blk[0][0] & 1u) ^ ((uint32_t)(tL >> level) & 1u & t);
+                const uint32_t tr_ = (blk[1][0] & 1u) ^ ((uint32_t)(tR >> level) & 1u & t);
+                const uint32_t r0 = (blk[1][0] & ~1u) ^ (cw[0] & mk), r1 = blk[1][1] ^ (cw[1] & mk);
+                const uint32_t r2 = blk[1][2] ^ (cw[2] & mk), r3 = blk[1][3] ^ (cw[3] & mk);
+#pragma unroll
+                for (int d = 0; d < kFullMaxDepth; d++)
+                    if (d == depth)
+                    {
+                        stS[d][0] = r0;
+                        stS[d][1] = r1;
+                        stS[d][2] = r2;
+                        stS[d][3] = r3;
+                        stT[d]    = tr_;
+                    }
+                s[0] = (blk[0][0] & ~1u) ^ (cw[0] & mk);
+                s[1] = blk[0][1] ^ (cw[1] & mk);
+                s[2] = blk[0][2] ^ (cw[2] & mk);
+                s[3] = blk[0][3] ^ (cw[3] & mk);
+                t    = tl_;
+                depth++;
+            }
+
+            const size_t j   = (size_t)((r << D) | leaf);
+            T           *dst = oe + ((xe - j) & (size - 1)) * (size_t)g;
+            if (g == 1)
+            {
+                uint32_t conv;
+                if (m <= 16)
+                    conv = s[0] & 0xffu;
+                else
+                {
+                    uint32_t b[1][4] = {{0u, 0u, 0u, 0u}};
+                    aes_encrypt<1>(c, s, b);
+                    conv = b[0][0] & 0xffu;
+                }
+                const T term = (T)((T)conv + (t ? tl[2] : (T)0));
+                dst[0]       = party ? (T)(0 - term) : term;
+            }
+            else
+            {
+                ConvCtx cv;
+                cv.init(s, m, g);
+                for (int jj = 0; jj < g; jj++)
+                {
+                    const T term = (T)((T)cv.at(c, jj) + (t ? tl[2 + jj] : (T)0));
+                    dst[jj]      = party ? (T)(0 - term) : term;
+                }
+            }
+
+            if (++leaf == leaves) break;
+            const int d0 = D - 1 - (int)__builtin_ctz(leaf);
+#pragma unroll
+            for (int d = 0; d < kFullMaxDepth; d++)
+                if (d == d0)
+                {
+                    s[0] = stS[d][0];
+                    s[1] = stS[d][1];
+                    s[2] = stS[d][2];
+                    s[3] = stS[d][3];
+                    t    = stT[d];
+                }
+            depth = d0 + 1;
+        }
+    }
+}
+
+// Subtree split of the full-domain evaluations (Grotto full domain, DPF
+// evalAll): enough lanes to fill every CU once (or the configured target),
+// and at most kFullMaxDepth levels expanded per lane.
 static std::atomic<size_t> g_full_lane_target{0};
 
-inline void grotto_full_split(int lv, size_t N, int &L, int &D)
+inline void full_domain_split(int lv, size_t N, int &L, int &D)
 {
     const size_t conf   = g_full_lane_target.load(std::memory_order_relaxed);
     const size_t target = conf ? conf : (size_t)device_cu_count() * kBlock;
@@ -574,11 +665,12 @@ extern "C" FAST_FSS_API int FastFss_cuda_dpfEvalAll(void *sharedOut, size_t shar
     if (cacheDataSize != 0 && cacheDataSize != dpf_cache_size(n, N)) return FAST_FSS_INVALID_CACHE_DATA_SIZE_ERROR;
     if (N == 0) return FAST_FSS_SUCCESS;
     if (!aligned(key, 8) || !aligned(seed, 8)) return FAST_FSS_RUNTIME_ERROR;
-    hipStream_t  st   = resolve_stream(cudaStreamPtr);
-    const size_t work = N << n;
-    FSS_DISPATCH(es, dpf_eval_all_kernel<T><<<grid_for(work), kBlock, 0, st>>>(
+    hipStream_t st = resolve_stream(cudaStreamPtr);
+    int         L, D;
+    full_domain_split((int)n, N, L, D);
+    FSS_DISPATCH(es, dpf_eval_all_dfs_kernel<T><<<grid_for(N << L), kBlock, 0, st>>>(
                          (T *)sharedOut, (const T *)maskedX, (const uint8_t *)key, (const uint8_t *)seed, partyId,
-                         (int)n, (int)m, (int)g, N));
+                         (int)n, (int)m, (int)g, L, D, N));
     return launch_status();
 }
 
@@ -772,7 +864,7 @@ extern "C" FAST_FSS_API int FastFss_hip_grottoEvalFullDomain(void *sharedOut, si
     if (N == 0) return FAST_FSS_SUCCESS;
     if (!aligned(key, 8) || !aligned(seed, 8) || !aligned(sharedOut, 16)) return FAST_FSS_RUNTIME_ERROR;
     int L, D;
-    grotto_full_split((int)n - 6, N, L, D);
+    full_domain_split((int)n - 6, N, L, D);
     hipStream_t st = resolve_stream(hipStreamPtr);
     FSS_DISPATCH(es, grotto_full_kernel<T><<<grid_for(N << L), kBlock, 0, st>>>(
                          sharedOut, (const uint8_t *)key, (const uint8_t *)seed, partyId, mode, packed, (int)n, L, D,

@@ -47,8 +47,9 @@ FAST_FSS_API int FastFss_hip_grottoEvalFullDomain(void *sharedOut, size_t shared
                                                   int partyId, int mode, int packed, size_t bitWidthIn,
                                                   size_t elementSize, size_t elementNum, void *hipStreamPtr);
 
-/* Number of lanes the subtree split aims for (0 restores the default, one
- * 1024-lane workgroup per CU).  Fewer lanes means deeper subtrees per lane
+/* Number of lanes the subtree split of the full-domain evaluations (this
+ * entry point and FastFss_cuda_dpfEvalAll) aims for; 0 restores the default,
+ * one 1024-lane workgroup per CU.  Fewer lanes means deeper subtrees per lane
  * (at most 8 levels); results do not depend on it.  Process-wide setting. */
 FAST_FSS_API int FastFss_hip_grottoFullSetLaneTarget(size_t lanes);
 

@@ -73,6 +73,28 @@ def test_dpf_eval_all_and_multi(dev, n, es, g):
                               dpf_eval_multi(OR, x, ko, sd, party, pts, n, m, g, es))
 
 
+@pytest.mark.parametrize("n,m,es,g,lanes", [(10, 8, 2, 1, 1), (10, 32, 4, 1, 16), (12, 16, 2, 1, 4),
+                                             (9, 20, 4, 3, 2), (14, 64, 8, 1, 1)])
+def test_dpf_eval_all_subtrees(dev, n, m, es, g, lanes):
+    """evalAll expands subtrees depth-first; small lane targets force deep
+    subtrees (up to 8 levels) so every stack depth is exercised."""
+    rng = np.random.default_rng(n * 7 + m + g)
+    N = 3
+    alpha = rand_elems(rng, N, n, es)
+    beta = rand_elems(rng, N * g, m, es)
+    x = rand_elems(rng, N, n, es)
+    s0, s1 = rand_seeds(rng, N), rand_seeds(rng, N)
+    ko = dpf_keygen(OR, alpha, beta, s0, s1, n, m, g, es)
+    kd = dev.put(ko)
+    dev.lib.FastFss_hip_grottoFullSetLaneTarget(lanes)
+    try:
+        for party, sd in ((0, s0), (1, s1)):
+            assert np.array_equal(dev.get(dpf_eval_all(dev, x, kd, sd, party, n, m, g, es)),
+                                  dpf_eval_all(OR, x, ko, sd, party, n, m, g, es)), party
+    finally:
+        dev.lib.FastFss_hip_grottoFullSetLaneTarget(0)
+
+
 @pytest.mark.parametrize("es,n,m", [(2, 16, 8), (4, 32, 32), (8, 64, 20)])
 def test_dpf_queue_size(dev, es, n, m):
     """From 2^16 elements the groupSize-1 DPF eval runs on the work queue; 70001
