@@ -351,7 +351,7 @@ def serving_leg(qps: float, requests: int):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
+        print(json.dumps({"error": "this benchmark measures one GPU; multi-GPU runs are not supported",
                           "n_gpus": a.gpus}))
         return 2
 
