@@ -316,10 +316,17 @@ __global__ __launch_bounds__(kBlock) void grotto_full_kernel(void *out, const ui
     const T            *tlr   = (const T *)k.tlr;
     const size_t        lanes = N << L;
     const int           lv    = n - 6; // tree levels above the leaf words
-    FSS_GRID_LOOP(w, lanes)
+    const uint32_t      lane  = threadIdx.x & 63u;
+    // whole waves iterate together (the element layout's stores are shared
+    // across the wave); a lane past the end redoes its wave's first subtree
+    // and writes nothing of its own
+    for (size_t wb = (size_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); wb < lanes;
+         wb += (size_t)gridDim.x * blockDim.x)
     {
-        const size_t   e  = w >> L;
-        const uint32_t r  = (uint32_t)(w & ((1u << L) - 1u));
+        const bool     valid = wb + lane < lanes;
+        const size_t   w     = valid ? wb + lane : wb;
+        const size_t   e     = w >> L;
+        const uint32_t r     = (uint32_t)(w & ((1u << L) - 1u));
         const uint8_t *sc = k.sCW + e * 16 * (size_t)lv;
         const T        tl = tlr[2 * e], tr = tlr[2 * e + 1];
         const uint64_t lastCW = k.lastCW[e];
@@ -381,18 +388,31 @@ __global__ __launch_bounds__(kBlock) void grotto_full_kernel(void *out, const ui
                 bits = prefix_parity64(mode == 1 ? (u << 1) : u);
                 if ((P ^ (uint32_t)party) & 1u) bits = ~bits;
             }
-            const size_t leafIdx = ((size_t)r << D) | leaf; // leaf word index within key e
             if (packed)
-                ((uint64_t *)out)[(e << lv) + leafIdx] = bits;
+            {
+                if (valid) ((uint64_t *)out)[(e << lv) + (((size_t)r << D) | leaf)] = bits;
+            }
             else
             {
-                uint8_t *dst = (uint8_t *)out + (((e << lv) + leafIdx) << 6) * sizeof(T);
-#pragma unroll
-                for (int q = 0; q < 4 * (int)sizeof(T); q++)
+                // The wave writes its 64 leaves' 64*sizeof(T)-byte runs together:
+                // store q covers chunks 64q .. 64q+63 of the wave's leaves in
+                // order, so each store fills whole lines of consecutive leaves'
+                // runs instead of one 16-byte piece of 64 scattered runs.
+                constexpr int CPL = 4 * (int)sizeof(T); // 16-byte chunks per leaf
+#pragma unroll 1
+                for (int q = 0; q < CPL; q++)
                 {
-                    uint32_t v[4];
-                    full_leaf_chunk<T>(v, bits, q);
-                    store16(dst + 16 * q, v);
+                    const uint32_t gci   = (uint32_t)q * 64u + lane;
+                    const uint32_t src   = gci / CPL, chunk = gci % CPL;
+                    const uint64_t sbits = __shfl(bits, (int)src);
+                    const size_t   sw    = wb + src;
+                    if (sw < lanes)
+                    {
+                        const size_t sLeaf = ((sw >> L) << lv) + (((sw & ((1u << L) - 1u)) << D) | leaf);
+                        uint32_t     v[4];
+                        full_leaf_chunk<T>(v, sbits, (int)chunk);
+                        store16_aligned((uint8_t *)out + ((sLeaf << 6) * sizeof(T)) + 16 * chunk, v);
+                    }
                 }
             }
 

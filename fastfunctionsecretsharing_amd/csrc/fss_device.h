@@ -525,6 +525,14 @@ __device__ __forceinline__ void store16(uint8_t *p, const uint32_t d[4])
     *(uint2 *)(p + 8) = make_uint2(d[2], d[3]);
 }
 
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// one 16-byte store to a 16-byte aligned address
+__device__ __forceinline__ void store16_aligned(uint8_t *p, const uint32_t d[4])
+{
+    *(u32x4 *)p = u32x4{d[0], d[1], d[2], d[3]};
+}
+
 template <typename T>
 __device__ __forceinline__ T mod_bits(T x, int bw)
 {
