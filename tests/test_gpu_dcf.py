@@ -118,6 +118,17 @@ def test_eval_variants_agree(dev, es, n, N):
     assert lib.FastFss_hip_setEvalVariant(lib.FastFss_hip_getEvalVariantCount()) == -1
 
 
+def test_dcf_cpu_workload_n16_2pow16(dev):
+    """The reference's CPU-sized workload (n = m = 16 in 64-bit elements, 2^16
+    elements, beta = NULL) on the GPU, bit-exact against the oracle."""
+    rng = np.random.default_rng(16)
+    N, n, m, es = 1 << 16, 16, 16, 8
+    alpha = rand_elems(rng, N, n, es)
+    x = rand_elems(rng, N, n, es)
+    s0, s1 = rand_seeds(rng, N), rand_seeds(rng, N)
+    _check(dev, alpha, None, x, s0, s1, n, m, 1, es)
+
+
 def test_dcf_2pow24_n32(dev):
     """Plain DCF, bitWidthIn = bitWidthOut = 32, 2^24 elements, one key each."""
     import torch

@@ -78,6 +78,23 @@ def test_dcf_reference_grid(es):
             assert np.array_equal(_rec(y0, y1, m, g), want), (n, m, g)
 
 
+def test_dcf_cpu_workload_n16_2pow16():
+    """The reference's CPU-sized DCF workload: bitWidthIn = bitWidthOut = 16,
+    2^16 elements stored in 64-bit elements, beta = NULL (= 1), alpha and the
+    masked input uniform over [0, 2^16); reconstruction (s0 + s1) mod 2^16 ==
+    [x < alpha]."""
+    rng = np.random.default_rng(16)
+    N, n, m, es = 1 << 16, 16, 16, 8
+    alpha = rand_elems(rng, N, n, es)
+    x = rand_elems(rng, N, n, es)
+    s0, s1 = rand_seeds(rng, N), rand_seeds(rng, N)
+    key = dcf_keygen(BE, alpha, None, s0, s1, n, m, 1, es)
+    assert key.nbytes == N * (16 * n + es * n + 3 * es)
+    y0 = dcf_eval(BE, x, key, s0, 0, n, m, 1, es)
+    y1 = dcf_eval(BE, x, key, s1, 1, n, m, 1, es)
+    assert np.array_equal(_rec(y0, y1, m, 1).reshape(-1), (x < alpha).astype(np.uint64))
+
+
 def test_dcf_beta_null_is_one():
     rng = np.random.default_rng(5)
     N, n, es = 64, 8, 1
