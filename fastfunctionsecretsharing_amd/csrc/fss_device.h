@@ -2,19 +2,21 @@
 //
 // AES-128 with a per-lane key (the PRG of every tree level is AES keyed by the
 // node seed, reference src/impl/dcf.h:423-424 / src/impl/aes.h:466-469), written
-// as a T-table cipher whose two tables live in LDS:
+// as a T-table cipher whose four tables live in LDS:
 //
 //   * State words are little-endian columns of the FIPS-197 byte array
 //     (byte r of column c sits in bits 8r..8r+7 of word c), so 16-byte seeds
 //     and blocks load and store with plain dword accesses.
-//   * TE0[x] = 2S(x) | S(x)<<8 | S(x)<<16 | 3S(x)<<24 and TE1 = rotl8(TE0).
-//     Rows 2 and 3 use rotl16 of rows 0 and 1, so a column costs 4 lookups,
-//     2 three-way XORs (v_bitop3_b32) and one v_alignbit_b32.
-//   * Each table entry is replicated 64 times with lane-private copies: entry
-//     x for lane L is at byte x*256 + (L%64)*4, so a wave's 64 lookups hit 64
-//     distinct banks whatever the data.  A single v_perm_b32 builds the LDS
-//     address from the state byte and a per-lane constant.  TE0 occupies
-//     bytes [0, 64K), TE1 bytes [64K, 128K): 128 KB per workgroup.
+//   * TE0[x] = 2S(x) | S(x)<<8 | S(x)<<16 | 3S(x)<<24 and TE_k = rotl(TE0, 8k),
+//     so a column costs 4 lookups and 2 three-way XORs (v_bitop3_b32), the
+//     round key folded into the second.
+//   * Each table entry is replicated 32 times with lane-private copies: entry
+//     x of table k for lane L is at byte x*256 + (k&1)*128 + (L%32)*4 +
+//     (k>>1)*64K.  A wave's lookups are served 32 lanes at a time and each
+//     half-wave hits 32 distinct banks whatever the data (measured: 32 and 64
+//     replicas give the same lookup rate).  A single v_perm_b32 builds the LDS
+//     address from the state byte and a per-lane, per-table constant.  The
+//     four tables take 128 KB per workgroup.
 //   * The last round and the key schedule take S(x) out of the table entries
 //     (S sits in bytes 1-2 of TE0 and bytes 2-3 of TE1).
 //
@@ -111,11 +113,6 @@ __device__ __forceinline__ uint32_t or_xor(uint32_t a, uint32_t b, uint32_t c)
     return __builtin_amdgcn_bitop3_b32(a, b, c, 0x56);
 }
 
-__device__ __forceinline__ uint32_t rot16(uint32_t a)
-{
-    return __builtin_amdgcn_alignbit(a, a, 16);
-}
-
 __device__ __forceinline__ uint32_t perm(uint32_t hi, uint32_t lo, uint32_t sel)
 {
     return __builtin_amdgcn_perm(hi, lo, sel);
@@ -129,41 +126,49 @@ __device__ __forceinline__ uint32_t perm(uint32_t hi, uint32_t lo, uint32_t sel)
 // workgroup, followed by __syncthreads().
 __device__ __forceinline__ void aes_fill_lds(uint32_t *lds)
 {
-    for (int w = threadIdx.x; w < 256 * 64; w += blockDim.x)
+    // word w: half (k>>1) = w>>14, entry x = (w>>6)&255, k&1 = (w>>5)&1, lane copy w&31
+    for (int w = threadIdx.x; w < kLdsTableWords; w += blockDim.x)
     {
-        uint32_t v  = kTe0.v[w >> 6];
-        lds[w]      = v;
-        lds[w + 16384] = __builtin_amdgcn_alignbit(v, v, 24); // rotl8
+        const uint32_t k = ((w >> 14) << 1) | ((w >> 5) & 1);
+        const uint32_t v = kTe0.v[(w >> 6) & 255];
+        lds[w]           = k ? __builtin_amdgcn_alignbit(v, v, 32 - 8 * k) : v; // rotl(v, 8k)
     }
 }
 
 struct AesCtx
 {
     const char *base; // LDS table base
-    uint32_t    la;   // (lane % 64) * 4            -> TE0 copy of this lane
-    uint32_t    lb;   // (lane % 64) * 4 | 0x10000  -> TE1 copy of this lane
+    uint32_t    l[4]; // byte offset of this lane's copy of table k (address bytes 0 and 2)
 };
 
 __device__ __forceinline__ AesCtx aes_ctx(const uint32_t *lds)
 {
-    AesCtx c;
-    c.base = (const char *)lds;
-    c.la   = (threadIdx.x & 63u) * 4u;
-    c.lb   = c.la | 0x10000u;
+    AesCtx         c;
+    const uint32_t lane = (threadIdx.x & 31u) * 4u;
+    c.base              = (const char *)lds;
+#pragma unroll
+    for (uint32_t k = 0; k < 4; k++) c.l[k] = lane | ((k & 1u) << 7) | ((k >> 1) << 16);
     return c;
 }
 
 // LDS byte address of the table entry for byte k of s: [lane byte, s.byte k, table, 0]
 #define FSS_SEL(k) (0x0C020000u | ((4u + (k)) << 8))
 
+// TE_t[byte k of s]
+template <int t>
+__device__ __forceinline__ uint32_t te(const AesCtx &c, uint32_t s, int k)
+{
+    return *(const uint32_t *)(c.base + perm(s, c.l[t], FSS_SEL(k)));
+}
+
 __device__ __forceinline__ uint32_t te0(const AesCtx &c, uint32_t s, int k)
 {
-    return *(const uint32_t *)(c.base + perm(s, c.la, FSS_SEL(k)));
+    return te<0>(c, s, k);
 }
 
 __device__ __forceinline__ uint32_t te1(const AesCtx &c, uint32_t s, int k)
 {
-    return *(const uint32_t *)(c.base + perm(s, c.lb, FSS_SEL(k)));
+    return te<1>(c, s, k);
 }
 
 // S-bytes of (a.b1, b.b1) into bytes 0,1 from two TE0 entries
@@ -182,13 +187,13 @@ __device__ __forceinline__ void aes_next_key(const AesCtx &c, uint32_t k[4], uin
     k[3] ^= k[2];
 }
 
-// one full round; h[c] = rot16(round key word c)
+// one full round; h[c] = round key word c
 __device__ __forceinline__ void aes_round(const AesCtx &c, uint32_t s[4], const uint32_t h[4])
 {
-    uint32_t n0 = xor3(te0(c, s[0], 0), te1(c, s[1], 1), rot16(xor3(te0(c, s[2], 2), te1(c, s[3], 3), h[0])));
-    uint32_t n1 = xor3(te0(c, s[1], 0), te1(c, s[2], 1), rot16(xor3(te0(c, s[3], 2), te1(c, s[0], 3), h[1])));
-    uint32_t n2 = xor3(te0(c, s[2], 0), te1(c, s[3], 1), rot16(xor3(te0(c, s[0], 2), te1(c, s[1], 3), h[2])));
-    uint32_t n3 = xor3(te0(c, s[3], 0), te1(c, s[0], 1), rot16(xor3(te0(c, s[1], 2), te1(c, s[2], 3), h[3])));
+    uint32_t n0 = xor3(te<0>(c, s[0], 0), te<1>(c, s[1], 1), xor3(te<2>(c, s[2], 2), te<3>(c, s[3], 3), h[0]));
+    uint32_t n1 = xor3(te<0>(c, s[1], 0), te<1>(c, s[2], 1), xor3(te<2>(c, s[3], 2), te<3>(c, s[0], 3), h[1]));
+    uint32_t n2 = xor3(te<0>(c, s[2], 0), te<1>(c, s[3], 1), xor3(te<2>(c, s[0], 2), te<3>(c, s[1], 3), h[2]));
+    uint32_t n3 = xor3(te<0>(c, s[3], 0), te<1>(c, s[0], 1), xor3(te<2>(c, s[1], 2), te<3>(c, s[2], 3), h[3]));
     s[0] = n0, s[1] = n1, s[2] = n2, s[3] = n3;
 }
 
@@ -224,7 +229,7 @@ __device__ __forceinline__ void aes_encrypt(const AesCtx &c, const uint32_t key[
     for (int r = 0; r < 9; r++)
     {
         aes_next_key(c, k, rcon[r]);
-        uint32_t h[4] = {rot16(k[0]), rot16(k[1]), rot16(k[2]), rot16(k[3])};
+        const uint32_t *h = k;
 #pragma unroll
         for (int b = 0; b < NB; b++) aes_round(c, st[b], h);
     }
@@ -247,7 +252,7 @@ __device__ __forceinline__ void aes_encrypt_upto(const AesCtx &c, const uint32_t
     for (int r = 0; r < 9; r++)
     {
         aes_next_key(c, k, rcon[r]);
-        uint32_t h[4] = {rot16(k[0]), rot16(k[1]), rot16(k[2]), rot16(k[3])};
+        const uint32_t *h = k;
 #pragma unroll
         for (int b = 0; b < NB; b++)
             if (b < nb) aes_round(c, st[b], h);
@@ -281,7 +286,7 @@ __device__ __forceinline__ void aes_encrypt_multi(const AesCtx &c, const uint32_
         for (int e = 0; e < E; e++)
         {
             aes_next_key(c, k[e], rcon[r]);
-            uint32_t h[4] = {rot16(k[e][0]), rot16(k[e][1]), rot16(k[e][2]), rot16(k[e][3])};
+            const uint32_t *h = k[e];
 #pragma unroll
             for (int b = 0; b < NB; b++) aes_round(c, st[e][b], h);
         }
