@@ -308,12 +308,13 @@ def piecewise_leg(reps: int = 3):
     return res
 
 
-def serving_leg(qps: float, requests: int):
+def serving_leg(qps: float, requests: int, sweep_qps=(200, 800, "burst")):
     """Poisson arrivals at `qps`; each request a DCF, MIC (16 intervals) or
     spline (degree 2, 4 intervals) evaluation of 2^10..2^20 elements
     (log-uniform), bitWidthIn in {16, 32, 64}; keys resident in HBM.  The same
     trace runs through the continuous-batching server and through the serial
-    per-call form (one request at a time, persistent per-call launch shape)."""
+    per-call form (one request at a time, persistent per-call launch shape), at
+    `qps` and at each offered load of `sweep_qps`."""
     import numpy as np
     import torch
 
@@ -327,24 +328,37 @@ def serving_leg(qps: float, requests: int):
     W.run_trace(templates, picks[:6], np.zeros(6), 1 << 35, 8, 4, 1)
     res = {"workload": "Poisson %.0f req/s, %d requests; DCF/MIC/spline 1:1:1, 2^10..2^20 elements log-uniform, "
                        "bitWidthIn in {16,32,64}; keys resident" % (qps, requests)}
-    outs = {}
-    for name, cfg in (("batched", (32 << 30, 16, 8, 1)), ("plain", (32 << 30, 1, 1, 0))):
-        o, lats, wall, st = W.run_trace(templates, picks, arrivals, *cfg)
-        outs[name] = o
-        elems = sum(templates[p].N for p in picks)
-        lat_ms = np.array(lats) * 1e3
-        res[name] = {
-            "evals_per_s": elems / wall,
-            "p50_ms": float(np.percentile(lat_ms, 50)),
-            "p99_ms": float(np.percentile(lat_ms, 99)),
-            "mean_ms": float(lat_ms.mean()),
-            "wall_s": wall,
-            "max_inflight": int(st.maxInflightSeen),
-            "config": {"hbm_budget_bytes": cfg[0], "max_inflight": cfg[1], "streams": cfg[2],
-                       "launch": "one workgroup per 1024 elements" if cfg[3] else "persistent, one workgroup per CU"},
-        }
-    res["outputs_equal"] = all(torch.equal(a, b) for a, b in zip(outs["batched"], outs["plain"]))
+    elems = sum(templates[p].N for p in picks)
+    forms = (("batched", (32 << 30, 16, 8, 1)), ("plain", (32 << 30, 1, 1, 0)))
+
+    def run(arr, detail):
+        out, row = {}, {}
+        for name, cfg in forms:
+            o, lats, wall, st = W.run_trace(templates, picks, arr, *cfg)
+            out[name] = o
+            lat_ms = np.array(lats) * 1e3
+            row[name] = {"evals_per_s": elems / wall, "p50_ms": float(np.percentile(lat_ms, 50)),
+                         "p99_ms": float(np.percentile(lat_ms, 99))}
+            if detail:
+                row[name].update({
+                    "mean_ms": float(lat_ms.mean()), "wall_s": wall, "max_inflight": int(st.maxInflightSeen),
+                    "config": {"hbm_budget_bytes": cfg[0], "max_inflight": cfg[1], "streams": cfg[2],
+                               "launch": "one workgroup per 1024 elements" if cfg[3]
+                               else "persistent, one workgroup per CU"}})
+        row["outputs_equal"] = all(torch.equal(a, b) for a, b in zip(out["batched"], out["plain"]))
+        return row
+
+    res.update(run(arrivals, True))
     res["p99_speedup"] = res["plain"]["p99_ms"] / res["batched"]["p99_ms"]
+    # the same requests at other offered loads; "burst" submits them all at once
+    sweep = []
+    unit = np.cumsum(rng.exponential(1.0, size=requests))
+    for q in sweep_qps:
+        arr = np.zeros(requests) if q == "burst" else unit / float(q)
+        row = run(arr, False)
+        row["qps"] = q
+        sweep.append(row)
+    res["qps_sweep"] = sweep
     return res
 
 
