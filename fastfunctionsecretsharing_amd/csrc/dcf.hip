@@ -32,8 +32,8 @@ __global__ __launch_bounds__(kBlock) void dcf_keygen_g1_kernel(uint8_t *key, con
     });
 }
 
-template <typename T>
-__global__ __launch_bounds__(kBlock) void dcf_keygen_kernel(uint8_t *key, const T *alpha, const T *beta,
+template <typename T, int OB>
+__global__ __launch_bounds__(kBlockGroupKeyGen) void dcf_keygen_kernel(uint8_t *key, const T *alpha, const T *beta,
                                                             const uint8_t *seed0, const uint8_t *seed1, int n, int m,
                                                             int g, size_t N)
 {
@@ -48,8 +48,8 @@ __global__ __launch_bounds__(kBlock) void dcf_keygen_kernel(uint8_t *key, const 
         uint32_t a[4], b[4];
         load16(a, seed0 + 16 * e);
         load16(b, seed1 + 16 * e);
-        dcf_keygen_one<T>(c, key + e * 16 * (size_t)n, (T *)tails + e * stride, alpha[e],
-                          beta ? beta + e * (size_t)g : nullptr, a, b, n, m, g);
+        dcf_keygen_group<T, OB>(c, key + e * 16 * (size_t)n, (T *)tails + e * stride, alpha[e],
+                                beta ? beta + e * (size_t)g : nullptr, a, b, n, m, g);
     }
 }
 
@@ -315,9 +315,12 @@ extern "C" FAST_FSS_API int FastFss_cuda_dcfKeyGen(void *key, size_t keyDataSize
     }
     else
     {
-        FSS_DISPATCH(es, dcf_keygen_kernel<T><<<grid_for(N), kBlock, 0, st>>>(
-                             (uint8_t *)key, (const T *)alpha, (const T *)b, (const uint8_t *)seed0,
-                             (const uint8_t *)seed1, (int)n, (int)m, (int)g, N));
+        const int ob = keygen_chunk_ob((int)m, es);
+        FSS_DISPATCH(es, FSS_KEYGEN_OB_DISPATCH(T, ob,
+                                                (dcf_keygen_kernel<T, OBV><<<grid_for(N), kBlockGroupKeyGen, 0, st>>>(
+                                                    (uint8_t *)key, (const T *)alpha, (const T *)b,
+                                                    (const uint8_t *)seed0, (const uint8_t *)seed1, (int)n, (int)m,
+                                                    (int)g, N))));
     }
     return launch_status();
 }

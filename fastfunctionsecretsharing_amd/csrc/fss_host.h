@@ -13,6 +13,10 @@ namespace fss {
 // Launches use one 1024-thread workgroup per CU: the AES tables take 128 KB
 // of the CU's 160 KB LDS, so a second workgroup would not fit anyway.
 constexpr int kBlock = 1024;
+// groupSize > 1 keyGen: 512 threads (two waves per SIMD with the one
+// table-holding workgroup per CU), so each lane has 256 registers for the
+// four value blocks and their conversion windows
+constexpr int kBlockGroupKeyGen = 512;
 
 int device_cu_count(); // cached per device
 

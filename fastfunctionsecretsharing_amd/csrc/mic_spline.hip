@@ -133,8 +133,8 @@ __device__ void spline_shift(T *outc, const T *coef, T shift, int d, int m)
     for (int i = 0; i <= d; i++) outc[i] = mod_bits<T>(outc[i], m);
 }
 
-template <typename T>
-__global__ __launch_bounds__(kBlock) void spline_keygen_kernel(uint8_t *key, T *eOut, T *betaOut, const T *alpha,
+template <typename T, int OB>
+__global__ __launch_bounds__(kBlockGroupKeyGen) void spline_keygen_kernel(uint8_t *key, T *eOut, T *betaOut, const T *alpha,
                                                                const uint8_t *seed0, const uint8_t *seed1,
                                                                const T *coef, int d, const T *left, const T *right,
                                                                size_t I, int n, int m, size_t N)
@@ -151,8 +151,8 @@ __global__ __launch_bounds__(kBlock) void spline_keygen_kernel(uint8_t *key, T *
         uint32_t a[4], b[4];
         load16(a, seed0 + 16 * e);
         load16(b, seed1 + 16 * e);
-        dcf_keygen_one<T>(c, key + e * 16 * (size_t)n, tails + e * (size_t)(n * g + 2 + g), (T)(al - 1), beta, a, b,
-                          n, m, (int)g);
+        dcf_keygen_group<T, OB>(c, key + e * 16 * (size_t)n, tails + e * (size_t)(n * g + 2 + g), (T)(al - 1), beta,
+                                a, b, n, m, (int)g);
         for (size_t q = 0; q < I; q++)
         {
             const T L = mod_bits<T>(left[q], n), R = mod_bits<T>(right[q], n);
@@ -199,28 +199,24 @@ __global__ __launch_bounds__(kBlock) void spline_eval_kernel(T *out, const T *x,
         const T       *tail = tails + e * (n * g + 2 + g);
         const T        xv   = mod_bits<T>(x[e], n);
         const T       *be = sBeta + e * g, *ee = sE + e * g;
-        T              w[8], carry[8];
+        // w[j] = sum over intervals of (cx*beta + e + nxt - cur)[j]; each walk's
+        // slots are added with their sign as soon as the walk ends
+        T w[8];
 #pragma unroll
-        for (int j = 0; j < 8; j++) w[j] = 0, carry[j] = 0;
+        for (int j = 0; j < 8; j++) w[j] = 0;
         for (size_t q = 0; q < I; q++)
         {
             const T L  = mod_bits<T>(left[q], n), R = mod_bits<T>(right[q], n);
             const T Rp = mod_bits<T>((T)(R + 1), n);
-            T       cur[8];
-            if (q > 0 && mod_bits<T>((T)(right[q - 1] + 1), n) == L)
+            T       acc[RS];
+            if (!(q > 0 && mod_bits<T>((T)(right[q - 1] + 1), n) == L))
             {
-#pragma unroll
-                for (int j = 0; j < 8; j++) cur[j] = carry[j];
-            }
-            else
-            {
-                T acc[RS];
                 spline_walk<T, OB>(c, acc, sc, tail, (T)(xv - 1 - L), s, party, n, m, (int)g, (int)(q * C), C);
 #pragma unroll
-                for (int j = 0; j < 8; j++) cur[j] = (j < RS) ? acc[j < RS ? j : 0] : (T)0;
+                for (int j = 0; j < 8; j++)
+                    if (j < C) w[j] = (T)(w[j] - acc[j < RS ? j : 0]);
             }
             const bool link = (q + 1 < I) && mod_bits<T>((T)(right[q] + 1), n) == mod_bits<T>(left[q + 1], n);
-            T          acc[RS];
             spline_walk<T, OB>(c, acc, sc, tail, (T)(xv - 1 - Rp), s, party, n, m, (int)g, (int)(q * C),
                                link ? 2 * C : C);
             const T cx = (T)((int)(xv > L) - (int)(xv > Rp));
@@ -228,16 +224,20 @@ __global__ __launch_bounds__(kBlock) void spline_eval_kernel(T *out, const T *x,
             for (int j = 0; j < 8; j++)
             {
                 if (j < C)
-                    w[j] = (T)(w[j] + (T)(cx * be[q * C + j]) - cur[j] + acc[j < RS ? j : 0] + ee[q * C + j]);
+                    w[j] = (T)(w[j] + (T)(cx * be[q * C + j]) + acc[j < RS ? j : 0] + ee[q * C + j]);
             }
-            // slots C..2C-1 of acc hold interval q+1's values at this point
-#pragma unroll
-            for (int j = 0; j < 8; j++)
+            // slots C..2C-1 hold interval q+1's values at its start point
+            if (link)
             {
-                T v = 0;
 #pragma unroll
-                for (int r = 0; r < RS; r++) v = (r == j + C) ? acc[r] : v;
-                carry[j] = v;
+                for (int j = 0; j < 8; j++)
+                {
+                    if (j >= C) break;
+                    T v = 0;
+#pragma unroll
+                    for (int r = 0; r < RS; r++) v = (r == j + C) ? acc[r] : v;
+                    w[j] = (T)(w[j] - v);
+                }
             }
         }
         T r = 0, pw = 1;
@@ -473,10 +473,13 @@ extern "C" FAST_FSS_API int FastFss_cuda_dcfSplineKeyGen(
     if (N == 0) return FAST_FSS_SUCCESS;
     if (!aligned(key, 8) || !aligned(seed0, 8) || !aligned(seed1, 8)) return FAST_FSS_RUNTIME_ERROR;
     hipStream_t st = resolve_stream(cudaStreamPtr);
-    FSS_DISPATCH(es, spline_keygen_kernel<T><<<grid_for(N), kBlock, 0, st>>>(
-                         (uint8_t *)key, (T *)e, (T *)beta, (const T *)alpha, (const uint8_t *)seed0,
-                         (const uint8_t *)seed1, (const T *)coefficients, (int)degree, (const T *)leftEndpoints,
-                         (const T *)rightEndpoints, I, (int)n, (int)m, N));
+    const int ob = keygen_chunk_ob((int)m, es);
+    FSS_DISPATCH(es, FSS_KEYGEN_OB_DISPATCH(
+                         T, ob,
+                         (spline_keygen_kernel<T, OBV><<<grid_for(N), kBlockGroupKeyGen, 0, st>>>(
+                             (uint8_t *)key, (T *)e, (T *)beta, (const T *)alpha, (const uint8_t *)seed0,
+                             (const uint8_t *)seed1, (const T *)coefficients, (int)degree, (const T *)leftEndpoints,
+                             (const T *)rightEndpoints, I, (int)n, (int)m, N))));
     return launch_status();
 }
 
