@@ -37,7 +37,7 @@ DPF_OPS_PER_LEVEL = 10 * 12 + 10 * 32
 BYTES_PER_EVAL_N32 = 652 + 16 + 4 + 4
 # HBM bytes per launch of the eval kernel at 2^24 elements, from the two
 # calibrated rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) recorded here
-TRAFFIC_PROFILE = REPO / "profiles" / "dcf_eval_queue_n32_2p24_traffic.md"
+TRAFFIC_PROFILE = REPO / "profiles" / "dcf_eval_queue_pre_n32_2p24_traffic.md"
 
 
 def profiled_traffic(log2n: int):
@@ -452,7 +452,7 @@ def main():
             "frac": achieved_tops / VALU_PEAK_TOPS,
             "traffic": profiled_traffic(a.log2n),
             "traffic_unit": "bytes/launch (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0, 1 GiB copy calibration)",
-            "kernel": "dcf_eval_queue_kernel<uint32_t, 8>",
+            "kernel": "dcf_eval_queue_kernel<uint32_t, 8, true>",
             "kernel_ms": kernel_ms,
             "ops_per_eval": n * OPS_PER_LEVEL,
             "hbm_achieved_gbs": hbm_gbs,

@@ -145,7 +145,7 @@ __global__ __launch_bounds__(kBlock) void dcf_eval_group_kernel(T *out, const T 
 // tile from a device-wide counter (lane L evaluates element 64*tile + L)
 // until the tiles run out, so waves that run fast take more tiles and the
 // launch ends when the work does, not when the slowest fixed share does.
-template <typename T, int CH>
+template <typename T, int CH, bool PRE>
 __global__ __launch_bounds__(kBlock) void dcf_eval_queue_kernel(T *out, const T *x, const uint8_t *key,
                                                                 const uint8_t *seed, int party, int n, int outBytes,
                                                                 size_t N, uint32_t *counter)
@@ -168,14 +168,23 @@ __global__ __launch_bounds__(kBlock) void dcf_eval_queue_kernel(T *out, const T 
         const size_t e = (size_t)tile * 64 + lane;
         if (e < N)
         {
-            const uint8_t *sc[1] = {key + e * sw};
-            const T       *tl[1] = {tails + e * ts};
-            const T        xv[1] = {x[e]};
-            uint32_t       s[1][4];
-            T              o[1];
-            load16(s[0], seed + 16 * e);
-            dcf_eval_g1_burst<T, CH, 1>(c, o, sc, tl, xv, s, party, n, outBytes);
-            out[e] = o[0];
+            if constexpr (PRE)
+            {
+                uint32_t s[4];
+                load16(s, seed + 16 * e);
+                out[e] = dcf_eval_g1_pre<T, CH>(c, key + e * sw, tails + e * ts, x[e], s, party, n, outBytes);
+            }
+            else
+            {
+                const uint8_t *sc[1] = {key + e * sw};
+                const T       *tl[1] = {tails + e * ts};
+                const T        xv[1] = {x[e]};
+                uint32_t       s[1][4];
+                T              o[1];
+                load16(s[0], seed + 16 * e);
+                dcf_eval_g1_burst<T, CH, 1>(c, o, sc, tl, xv, s, party, n, outBytes);
+                out[e] = o[0];
+            }
         }
     }
 }
@@ -202,12 +211,14 @@ void queue_counter_release(hipStream_t st, uint32_t *p)
 
 // Groupsize-1 evaluation variants: 0 = per-level loads, 1..4 = bursts of
 // CH levels with E elements per lane (see dcf_eval_g1_burst) over a static
-// grid-stride split, 5 = CH 8 bursts over the work queue.
+// grid-stride split, 5 = CH 8 bursts over the work queue, 6 = variant 5 with
+// each element's value corrections loaded up front (dcf_eval_g1_pre; elements
+// of at most 32 bits, 64-bit elements run variant 5).
 struct EvalVariant
 {
     int ch, e, queue;
 };
-constexpr EvalVariant kEvalVariants[] = {{1, 1, 0}, {4, 1, 0}, {8, 1, 0}, {4, 2, 0}, {2, 2, 0}, {8, 1, 1}};
+constexpr EvalVariant kEvalVariants[] = {{1, 1, 0}, {4, 1, 0}, {8, 1, 0}, {4, 2, 0}, {2, 2, 0}, {8, 1, 1}, {8, 1, 2}};
 constexpr int         kNumEvalVariants = sizeof(kEvalVariants) / sizeof(kEvalVariants[0]);
 constexpr size_t      kQueueMinElems   = (size_t)1 << 16; // below this the static split is used
 
@@ -220,19 +231,26 @@ int dcf_eval_variant(size_t es, size_t n, size_t N)
     if (g_variant_override >= 0) return g_variant_override;
     int v = (es < 9) ? g_variant_table[es] : -1;
     if (v >= 0) return v;
-    // measured on MI355X: the queue form is 9-16% faster on large launches
-    return N >= kQueueMinElems ? 5 : 2;
+    // measured on MI355X: the queue form is 9-16% faster on large launches,
+    // and preloading the value corrections reads each key byte once (n = 32,
+    // 2^24: 11.3 instead of 19.0 GB fetched, 1-2% faster)
+    return N >= kQueueMinElems ? 6 : 2;
 }
 
 template <typename T>
 int launch_dcf_eval_g1(int v, int grid, hipStream_t st, T *out, const T *x, const uint8_t *key, const uint8_t *seed,
                        int party, int n, int outBytes, size_t N)
 {
-    if (v == 5)
+    if (v == 5 || v == 6)
     {
         uint32_t *counter = queue_counter_acquire(st);
         if (!counter) return FAST_FSS_RUNTIME_ERROR;
-        dcf_eval_queue_kernel<T, 8><<<grid, kBlock, 0, st>>>(out, x, key, seed, party, n, outBytes, N, counter);
+        if (v == 6 && sizeof(T) <= 4)
+            dcf_eval_queue_kernel<T, 8, (sizeof(T) <= 4)>
+                <<<grid, kBlock, 0, st>>>(out, x, key, seed, party, n, outBytes, N, counter);
+        else
+            dcf_eval_queue_kernel<T, 8, false>
+                <<<grid, kBlock, 0, st>>>(out, x, key, seed, party, n, outBytes, N, counter);
         queue_counter_release(st, counter);
         return FAST_FSS_SUCCESS;
     }
@@ -248,12 +266,12 @@ int launch_dcf_eval_g1(int v, int grid, hipStream_t st, T *out, const T *x, cons
 }
 
 // Serving launches choose their own grid shape, so they keep the static split
-// (variant 5 falls back to variant 2 there).
+// (variants 5 and 6 fall back to variant 2 there).
 int launch_dcf_eval_g1_rt(size_t es, int grid, hipStream_t st, void *out, const void *x, const void *key,
                           const void *seed, int party, int n, int m, size_t N)
 {
     int v = dcf_eval_variant(es, (size_t)n, N);
-    if (v == 5) v = 2;
+    if (v == 5 || v == 6) v = 2;
     int rc = FAST_FSS_SUCCESS;
     FSS_DISPATCH(es, rc = launch_dcf_eval_g1<T>(v, grid, st, (T *)out, (const T *)x, (const uint8_t *)key,
                                                 (const uint8_t *)seed, party, n, (m + 7) / 8, N));

@@ -149,6 +149,65 @@ __device__ __forceinline__ void dcf_eval_g1_burst(const AesCtx &c, T out[E], con
     }
 }
 
+// dcf_eval_g1_burst for one element whose value corrections (n of them, plus
+// tLCW, tRCW, lastCW) are all loaded when the walk starts, for elements of at
+// most 32 bits.  A wave's tails are then read once, as whole lines, instead
+// of once per burst: the bursts' 4-byte-per-level reads of neighbouring
+// 4(n+3)-byte tails touch the same lines again after they have left the
+// cache.  The seed corrections still come in bursts of CH levels (one 128-B
+// line per burst at CH = 8).  The level loop is not unrolled, so the window
+// of corrections shifts down by CH after each burst to keep register indices
+// static.  Results equal dcf_eval_g1.
+template <typename T, int CH>
+__device__ __forceinline__ T dcf_eval_g1_pre(const AesCtx &c, const uint8_t *sCW, const T *tail, T xin,
+                                             uint32_t s[4], int party, int n, int outBytes)
+{
+    static_assert(sizeof(T) <= 4, "preloaded corrections: elements of at most 32 bits");
+    constexpr int NMAX = 8 * sizeof(T);
+    T             vall[NMAX];
+#pragma unroll
+    for (int j = 0; j < NMAX; j++) vall[j] = (j < n) ? tail[j] : (T)0;
+    const T tl = tail[n], tr = tail[n + 1], lastCW = tail[n + 2];
+    s[0] &= ~1u;
+    uint32_t t   = (uint32_t)party;
+    const T  x   = mod_bits<T>(xin, n);
+    T        acc = 0;
+    for (int i0 = 0; i0 < n; i0 += CH)
+    {
+        uint32_t cw[CH][4];
+#pragma unroll
+        for (int k = 0; k < CH; k++)
+            if (i0 + k < n) load16(cw[k], sCW + 16 * (i0 + k));
+#pragma unroll
+        for (int k = 0; k < CH; k++)
+        {
+            const int i = i0 + k;
+            if (i < n)
+            {
+                const uint32_t xi = (uint32_t)(x >> (n - 1 - i)) & 1u;
+                uint32_t       blk[2][4];
+                blk[0][0] = 4u * xi, blk[0][1] = 0u, blk[0][2] = 4u * xi + 1u, blk[0][3] = 0u;
+                blk[1][0] = 4u * xi + 2u, blk[1][1] = 0u, blk[1][2] = 4u * xi + 3u, blk[1][3] = 0u;
+                aes_encrypt<2>(c, s, blk);
+                const uint32_t m  = 0u - t;
+                uint32_t       tn = blk[0][0] & 1u;
+                s[0]              = (blk[0][0] & ~1u) ^ (cw[k][0] & m);
+                s[1]              = blk[0][1] ^ (cw[k][1] & m);
+                s[2]              = blk[0][2] ^ (cw[k][2] & m);
+                s[3]              = blk[0][3] ^ (cw[k][3] & m);
+                tn ^= (uint32_t)(((xi ? tr : tl) >> i) & 1) & t;
+                T term = (T)((T)low_bytes(blk[1], outBytes) + (t ? vall[k] : (T)0));
+                acc    = (T)(acc + neg_if<T>(term, party));
+                t      = tn;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < NMAX; j++) vall[j] = (j + CH < NMAX) ? vall[j + CH] : (T)0;
+    }
+    T term = (T)((T)low_bytes(s, outBytes) + (t ? lastCW : (T)0));
+    return (T)(acc + neg_if<T>(term, party));
+}
+
 // Share for group elements [j0, j0 + cnt) of a groupSize-g DCF; cnt <= R.
 // Used for g > 1 (R covers the group) and by the spline evaluator, which
 // needs only the coefficient ranges of one or two intervals.
