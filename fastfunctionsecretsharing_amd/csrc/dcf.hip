@@ -27,7 +27,7 @@ __global__ __launch_bounds__(kBlock) void dcf_keygen_g1_kernel(uint8_t *key, con
         uint32_t a[4], b[4];
         load16(a, seed0 + 16 * e);
         load16(b, seed1 + 16 * e);
-        dcf_keygen_g1<T, 4>(c, key + e * 16 * (size_t)n, (T *)tails + e * stride, alpha[e], beta ? beta[e] : (T)1,
+        dcf_keygen_g1<T, 8>(c, key + e * 16 * (size_t)n, (T *)tails + e * stride, alpha[e], beta ? beta[e] : (T)1,
                             a, b, n, m);
     });
 }

@@ -24,17 +24,19 @@ __device__ __forceinline__ void tree_keygen_level(const AesCtx &c, uint32_t s0[4
     aes_encrypt<2>(c, s1, b1);
     const uint32_t tL0 = b0[0][0] & 1u, tR0 = b0[1][0] & 1u, tL1 = b1[0][0] & 1u, tR1 = b1[1][0] & 1u;
     b0[0][0] &= ~1u, b0[1][0] &= ~1u, b1[0][0] &= ~1u, b1[1][0] &= ~1u;
-    const int lose = 1 - keep;
+    // children chosen by masks: a selected array index would place b0/b1 in
+    // scratch memory
+    const uint32_t km = 0u - (uint32_t)keep;
 #pragma unroll
-    for (int k = 0; k < 4; k++) cw[k] = lose ? (b0[1][k] ^ b1[1][k]) : (b0[0][k] ^ b1[0][k]);
-    tLCW               = tL0 ^ tL1 ^ (uint32_t)keep ^ 1u;
-    tRCW               = tR0 ^ tR1 ^ (uint32_t)keep;
+    for (int k = 0; k < 4; k++) cw[k] = ((b0[0][k] ^ b1[0][k]) & km) | ((b0[1][k] ^ b1[1][k]) & ~km);
+    tLCW              = tL0 ^ tL1 ^ (uint32_t)keep ^ 1u;
+    tRCW              = tR0 ^ tR1 ^ (uint32_t)keep;
     const uint32_t m0 = 0u - t0, m1 = 0u - t1;
 #pragma unroll
     for (int k = 0; k < 4; k++)
     {
-        s0[k] = (keep ? b0[1][k] : b0[0][k]) ^ (cw[k] & m0);
-        s1[k] = (keep ? b1[1][k] : b1[0][k]) ^ (cw[k] & m1);
+        s0[k] = ((b0[1][k] & km) | (b0[0][k] & ~km)) ^ (cw[k] & m0);
+        s1[k] = ((b1[1][k] & km) | (b1[0][k] & ~km)) ^ (cw[k] & m1);
     }
     const uint32_t tk0 = keep ? tR0 : tL0, tk1 = keep ? tR1 : tL1, tkc = keep ? tRCW : tLCW;
     t0 = tk0 ^ (t0 & tkc);
@@ -71,13 +73,26 @@ __device__ void dpf_keygen_one(const AesCtx &c, uint8_t *sCW, T *tail, T alpha, 
     uint32_t s1[4] = {seed1[0] & ~1u, seed1[1], seed1[2], seed1[3]};
     uint32_t t0 = 0, t1 = 1;
     T        tl = 0, tr = 0;
-    for (int i = 0; i < n; i++)
+    // the correction words of 8 levels (one 128-B line) are stored together,
+    // so each line is written whole rather than 16 B at a time
+    for (int i0 = 0; i0 < n; i0 += 8)
     {
-        uint32_t cw[4], tLCW, tRCW;
-        tree_keygen_level(c, s0, s1, t0, t1, (int)((alpha >> (n - 1 - i)) & 1), cw, tLCW, tRCW);
-        store16(sCW + 16 * i, cw);
-        tl |= (T)((T)tLCW << i);
-        tr |= (T)((T)tRCW << i);
+        uint32_t cw[8][4];
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+        {
+            const int i = i0 + k;
+            if (i < n)
+            {
+                uint32_t tLCW, tRCW;
+                tree_keygen_level(c, s0, s1, t0, t1, (int)((alpha >> (n - 1 - i)) & 1), cw[k], tLCW, tRCW);
+                tl |= (T)((T)tLCW << i);
+                tr |= (T)((T)tRCW << i);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            if (i0 + k < n) store16(sCW + 16 * (i0 + k), cw[k]);
     }
     tail[0] = tl;
     tail[1] = tr;

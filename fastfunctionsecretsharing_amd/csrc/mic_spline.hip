@@ -41,7 +41,7 @@ __global__ __launch_bounds__(kBlock) void mic_keygen_kernel(uint8_t *key, T *z, 
         load16(b, seed1 + 16 * e);
         const T al = alpha[e];
         // DCF on alpha - 1 with beta = 1 (src/impl/mic.h:22)
-        dcf_keygen_g1<T, 4>(c, key + e * 16 * (size_t)n, tails + e * (size_t)(n + 3), (T)(al - 1), (T)1, a, b, n, m);
+        dcf_keygen_g1<T, 8>(c, key + e * 16 * (size_t)n, tails + e * (size_t)(n + 3), (T)(al - 1), (T)1, a, b, n, m);
         for (size_t i = 0; i < I; i++)
         {
             const T qP = mod_bits<T>((T)(right[i] + 1), n);
