@@ -97,11 +97,13 @@ def _sub_key(key, idx, n, g, es, N):
     return np.concatenate([sc.reshape(-1), tl.reshape(-1)])
 
 
-@pytest.mark.parametrize("es,n", [(1, 8), (2, 13), (4, 32), (8, 64)])
+@pytest.mark.parametrize("es,n", [(1, 8), (1, 3), (2, 13), (4, 32), (4, 27), (4, 5), (8, 64)])
 @pytest.mark.parametrize("N", [5000, 70001])
 def test_eval_variants_agree(dev, es, n, N):
     """Every groupSize-1 launch variant (static bursts, elements per lane, the
-    work queue) gives the oracle's shares; 70001 elements end in a partial tile."""
+    work queue, preloaded value corrections) gives the oracle's shares; 70001
+    elements end in a partial tile, and n = 3, 5, 13, 27 end in a partial
+    8-level burst."""
     rng = np.random.default_rng(es * 1000 + N)
     alpha, beta, masked, s0, s1 = _case(rng, N, n, n, 1, es)
     key_o = dcf_keygen(OR, alpha, beta, s0, s1, n, n, 1, es)
