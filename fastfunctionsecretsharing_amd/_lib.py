@@ -87,6 +87,7 @@ SIGNATURES = {
     "FastFss_hip_serverSubmit": ("ppp", "int"),
     "FastFss_hip_serverQuery": ("pq", "int"),
     "FastFss_hip_serverWait": ("pqp", "int"),
+    "FastFss_hip_serverRelease": ("pq", "int"),
     "FastFss_hip_serverDrain": ("p", "int"),
     "FastFss_hip_serverStats": ("pp", "int"),
     "FastFss_hip_serverDestroy": ("p", None),

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <mutex>
 #include <numeric>
 #include <vector>
@@ -229,7 +230,7 @@ struct BatchPlan
 static std::mutex g_tile_mu;
 // default elements per tile, [op][elementSize]
 static uint32_t g_tile[2][9] = {{0, 256, 256, 0, 128, 0, 0, 0, 64}, {0, 512, 256, 0, 256, 0, 0, 0, 128}};
-static int      g_blocks_per_cu = 1;
+static std::atomic<int> g_blocks_per_cu{1};
 
 static int validate(const FastFssBatchGroup &q)
 {
@@ -261,7 +262,7 @@ static int plan_enqueue(BatchPlan *p, hipStream_t st)
     batch_reset_kernel<<<1, 1, 0, st>>>(p->counter);
     if (p->tiles == 0) return launch_status();
     const int cus  = device_cu_count();
-    size_t    grid = (size_t)cus * (size_t)g_blocks_per_cu;
+    size_t    grid = (size_t)cus * (size_t)g_blocks_per_cu.load(std::memory_order_relaxed);
     // never launch more workgroups than 16 tiles each can keep busy
     const size_t need = (p->tiles + 15) / 16;
     if (grid > need) grid = need < 1 ? 1 : need;
@@ -292,7 +293,7 @@ extern "C" FAST_FSS_API uint32_t FastFss_hip_batchGetDefaultTile(int op, size_t 
 extern "C" FAST_FSS_API int FastFss_hip_batchSetBlocksPerCu(int blocks)
 {
     if (blocks < 1 || blocks > 8) return FAST_FSS_RUNTIME_ERROR;
-    g_blocks_per_cu = blocks;
+    g_blocks_per_cu.store(blocks, std::memory_order_relaxed);
     return FAST_FSS_SUCCESS;
 }
 

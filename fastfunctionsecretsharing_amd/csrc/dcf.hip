@@ -5,6 +5,8 @@
 // iteration of a grid-stride loop.  Argument validation follows the
 // reference Task::check order (src/kernel/dcf.h:31-69, 115-163) so callers see
 // the same error codes.
+#include <atomic>
+
 #include "FastFss/cuda/dcf.h"
 #include "dcf_device.h"
 #include "fss_host.h"
@@ -222,14 +224,17 @@ constexpr EvalVariant kEvalVariants[] = {{1, 1, 0}, {4, 1, 0}, {8, 1, 0}, {4, 2,
 constexpr int         kNumEvalVariants = sizeof(kEvalVariants) / sizeof(kEvalVariants[0]);
 constexpr size_t      kQueueMinElems   = (size_t)1 << 16; // below this the static split is used
 
-static int g_variant_override = -1; // FastFss_hip_setEvalVariant
-static int g_variant_table[9] = {-1, -1, -1, -1, -1, -1, -1, -1, -1}; // per element size (-1: default)
+// set from any host thread while others launch: atomics, relaxed (each value
+// is independent and only steers the choice of launch)
+static std::atomic<int> g_variant_override{-1};                             // FastFss_hip_setEvalVariant
+static std::atomic<int> g_variant_table[9] = {-1, -1, -1, -1, -1, -1, -1, -1, -1}; // per element size (-1: default)
 
 int dcf_eval_variant(size_t es, size_t n, size_t N)
 {
     (void)n;
-    if (g_variant_override >= 0) return g_variant_override;
-    int v = (es < 9) ? g_variant_table[es] : -1;
+    const int o = g_variant_override.load(std::memory_order_relaxed);
+    if (o >= 0) return o;
+    int v = (es < 9) ? g_variant_table[es].load(std::memory_order_relaxed) : -1;
     if (v >= 0) return v;
     // measured on MI355X: the queue form is 9-16% faster on large launches,
     // and preloading the value corrections reads each key byte once (n = 32,
@@ -285,7 +290,7 @@ using namespace fss;
 extern "C" FAST_FSS_API int FastFss_hip_setEvalVariant(int variant)
 {
     if (variant < -1 || variant >= kNumEvalVariants) return FAST_FSS_RUNTIME_ERROR;
-    g_variant_override = variant;
+    g_variant_override.store(variant, std::memory_order_relaxed);
     return FAST_FSS_SUCCESS;
 }
 
@@ -298,7 +303,7 @@ extern "C" FAST_FSS_API int FastFss_hip_setEvalVariantForElementSize(size_t elem
 {
     if (elementSize >= 9 || !elem_size_ok(elementSize) || variant < -1 || variant >= kNumEvalVariants)
         return FAST_FSS_RUNTIME_ERROR;
-    g_variant_table[elementSize] = variant;
+    g_variant_table[elementSize].store(variant, std::memory_order_relaxed);
     return FAST_FSS_SUCCESS;
 }
 

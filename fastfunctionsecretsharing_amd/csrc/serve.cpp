@@ -276,6 +276,17 @@ extern "C" FAST_FSS_API int FastFss_hip_serverWait(void *server, uint64_t ticket
     return it->status;
 }
 
+extern "C" FAST_FSS_API int FastFss_hip_serverRelease(void *server, uint64_t ticket)
+{
+    Server *s = (Server *)server;
+    if (!s) return FAST_FSS_RUNTIME_ERROR;
+    std::lock_guard<std::mutex> lock(s->mu);
+    auto                        f = s->items.find(ticket);
+    if (f == s->items.end() || f->second->status > 0) return FAST_FSS_RUNTIME_ERROR;
+    s->items.erase(f);
+    return FAST_FSS_SUCCESS;
+}
+
 extern "C" FAST_FSS_API int FastFss_hip_serverDrain(void *server)
 {
     Server *s = (Server *)server;

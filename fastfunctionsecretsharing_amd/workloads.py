@@ -346,6 +346,7 @@ def run_trace(templates, picks, arrivals, budget, max_inflight, streams, launch_
     for tk in tickets:
         lat = ctypes.c_double()
         assert lib.FastFss_hip_serverWait(srv, tk, ctypes.byref(lat)) == 0
+        assert lib.FastFss_hip_serverRelease(srv, tk) == 0
         lats.append(lat.value)
     st = _lib.ServeStats()
     lib.FastFss_hip_serverStats(srv, ctypes.byref(st))

@@ -83,6 +83,13 @@ FAST_FSS_API int FastFss_hip_serverQuery(void *server, uint64_t ticket);
  * is not NULL, its submit-to-completion latency. */
 FAST_FSS_API int FastFss_hip_serverWait(void *server, uint64_t ticket, double *latencySeconds);
 
+/* Forget a finished (done or failed) request: its ticket becomes unknown to
+ * Query and Wait.  A request's record otherwise stays until
+ * FastFss_hip_serverDestroy, so a long-running caller releases each ticket
+ * once it has the result.  Returns FAST_FSS_RUNTIME_ERROR for an unknown
+ * ticket or one still queued or running. */
+FAST_FSS_API int FastFss_hip_serverRelease(void *server, uint64_t ticket);
+
 /* Block until every submitted request is done. */
 FAST_FSS_API int FastFss_hip_serverDrain(void *server);
 

@@ -81,4 +81,9 @@ def test_submit_validation(env):
     assert lib.FastFss_hip_serverSubmit(srv, ctypes.byref(r), ctypes.byref(tk)) == 0
     assert lib.FastFss_hip_serverWait(srv, tk.value, None) == 0
     assert lib.FastFss_hip_serverQuery(srv, tk.value) == 1
+    # a released ticket is forgotten; releasing it again fails
+    assert lib.FastFss_hip_serverRelease(srv, tk.value) == 0
+    assert lib.FastFss_hip_serverQuery(srv, tk.value) == -1
+    assert lib.FastFss_hip_serverRelease(srv, tk.value) == -1
+    assert lib.FastFss_hip_serverRelease(srv, tk.value + 1000) == -1
     lib.FastFss_hip_serverDestroy(srv)
