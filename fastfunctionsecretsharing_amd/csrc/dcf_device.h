@@ -577,11 +577,15 @@ __device__ __forceinline__ void convert_chunk_pair(const AesCtx &c, uint32_t out
     aes_encrypt_multi<2, NWC>(c, keys, out);
 }
 
-// dcf_keygen_one for OB = ceil(bitWidthOut/8) in {1, 2, 4, 8} (compile time):
-// each level's four value conversions (both parties, lost and kept child) run
-// in chunks of NWC windows = SPC slots that share one key schedule per key,
-// and the two keys a difference needs are encrypted together.  Same key bytes
-// as dcf_keygen_one (reference src/impl/dcf.h:187-368).
+// dcf_keygen_one for OB = ceil(bitWidthOut/8) in {1, 2, 4, 8} (compile time),
+// slot chunk by slot chunk: the tree walk is repeated for each chunk of SPC
+// group slots (NWC conversion windows), so the chunk's running lastCW values
+// stay in registers for the whole walk and every value correction is written
+// once, instead of reading and rewriting all g lastCW values at every level.
+// The first pass also writes the seed and control-bit corrections.  Each
+// chunk's windows share one key schedule per key, and the two keys a
+// difference needs are encrypted together.  Same key bytes as dcf_keygen_one
+// (reference src/impl/dcf.h:187-368).
 template <typename T, int OB>
 __device__ void dcf_keygen_chunked(const AesCtx &c, uint8_t *sCW, T *tail, T alpha, const T *beta,
                                    const uint32_t seed0[4], const uint32_t seed1[4], int n, int g)
@@ -590,113 +594,106 @@ __device__ void dcf_keygen_chunked(const AesCtx &c, uint8_t *sCW, T *tail, T alp
     constexpr int SPC    = 16 * NWC / OB;
     const bool    direct = OB * g <= 16;
     T            *vcw    = tail;
-    T            *last   = tail + (size_t)n * g + 2;
-    for (int j = 0; j < g; j++) last[j] = 0;
-    uint32_t s0[4] = {seed0[0] & ~1u, seed0[1], seed0[2], seed0[3]};
-    uint32_t s1[4] = {seed1[0] & ~1u, seed1[1], seed1[2], seed1[3]};
-    uint32_t t0 = 0, t1 = 1;
-    T        tlBits = 0, trBits = 0;
-    for (int i = 0; i < n; i++)
+    T            *lastCW = tail + (size_t)n * g + 2;
+    for (int j0 = 0; j0 < g; j0 += SPC)
     {
-        const int keep = (int)((alpha >> (n - 1 - i)) & 1);
-        uint32_t  vl0[4], vl1[4], vk0[4], vk1[4];
-        const T   sign1 = t1 ? (T)-1 : (T)1;
+        const bool first = j0 == 0;
+        T          last[SPC], bj[SPC];
+#pragma unroll
+        for (int r = 0; r < SPC; r++)
         {
-            // per party: the lost seed, kept seed, both value blocks and the
-            // t bits
-            const uint32_t km = 0u - (uint32_t)keep;
-            uint32_t       sl[2][4], sk[2][4], tl[2], tr[2];
-#pragma unroll
-            for (int p = 0; p < 2; p++)
+            const int j = j0 + r;
+            last[r]     = 0;
+            bj[r]       = (j < g) ? (beta ? beta[j] : (T)(j == 0 ? 1 : 0)) : (T)0;
+        }
+        uint32_t s0[4] = {seed0[0] & ~1u, seed0[1], seed0[2], seed0[3]};
+        uint32_t s1[4] = {seed1[0] & ~1u, seed1[1], seed1[2], seed1[3]};
+        uint32_t t0 = 0, t1 = 1;
+        T        tlBits = 0, trBits = 0;
+        for (int i = 0; i < n; i++)
+        {
+            const int keep = (int)((alpha >> (n - 1 - i)) & 1);
+            uint32_t  vl0[4], vl1[4], vk0[4], vk1[4];
+            const T   sign1 = t1 ? (T)-1 : (T)1;
             {
-                const uint32_t *sp = p ? s1 : s0;
-                uint32_t        b[4][4];
+                // per party: the lost seed, kept seed, both value blocks and
+                // the t bits
+                const uint32_t km = 0u - (uint32_t)keep;
+                uint32_t       sl[2][4], sk[2][4], tl[2], tr[2];
 #pragma unroll
-                for (int q = 0; q < 4; q++) prg_pt(b[q], q);
-                aes_encrypt<4>(c, sp, b);
-                tl[p] = b[0][0] & 1u, tr[p] = b[2][0] & 1u;
-                b[0][0] &= ~1u, b[2][0] &= ~1u;
-                uint32_t *vl = p ? vl1 : vl0, *vk = p ? vk1 : vk0;
+                for (int p = 0; p < 2; p++)
+                {
+                    const uint32_t *sp = p ? s1 : s0;
+                    uint32_t        b[4][4];
+#pragma unroll
+                    for (int q = 0; q < 4; q++) prg_pt(b[q], q);
+                    aes_encrypt<4>(c, sp, b);
+                    tl[p] = b[0][0] & 1u, tr[p] = b[2][0] & 1u;
+                    b[0][0] &= ~1u, b[2][0] &= ~1u;
+                    uint32_t *vl = p ? vl1 : vl0, *vk = p ? vk1 : vk0;
+#pragma unroll
+                    for (int k = 0; k < 4; k++)
+                    {
+                        // blocks: 0 = left seed, 1 = left value, 2 = right seed, 3 = right value
+                        sk[p][k] = (b[2][k] & km) | (b[0][k] & ~km);
+                        sl[p][k] = (b[0][k] & km) | (b[2][k] & ~km);
+                        vl[k]    = (b[1][k] & km) | (b[3][k] & ~km);
+                        vk[k]    = (b[3][k] & km) | (b[1][k] & ~km);
+                    }
+                }
+                uint32_t cw[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) cw[k] = sl[0][k] ^ sl[1][k];
+                if (first) store16(sCW + 16 * i, cw);
+                const uint32_t tLCW = tl[0] ^ tl[1] ^ (uint32_t)keep ^ 1u;
+                const uint32_t tRCW = tr[0] ^ tr[1] ^ (uint32_t)keep;
+                tlBits |= (T)((T)tLCW << i);
+                trBits |= (T)((T)tRCW << i);
+                const uint32_t m0 = 0u - t0, m1 = 0u - t1;
 #pragma unroll
                 for (int k = 0; k < 4; k++)
                 {
-                    // blocks: 0 = left seed, 1 = left value, 2 = right seed, 3 = right value
-                    sk[p][k] = (b[2][k] & km) | (b[0][k] & ~km);
-                    sl[p][k] = (b[0][k] & km) | (b[2][k] & ~km);
-                    vl[k]    = (b[1][k] & km) | (b[3][k] & ~km);
-                    vk[k]    = (b[3][k] & km) | (b[1][k] & ~km);
+                    s0[k] = sk[0][k] ^ (cw[k] & m0);
+                    s1[k] = sk[1][k] ^ (cw[k] & m1);
                 }
+                const uint32_t tk0 = keep ? tr[0] : tl[0], tk1 = keep ? tr[1] : tl[1], tkc = keep ? tRCW : tLCW;
+                t0 = tk0 ^ (t0 & tkc);
+                t1 = tk1 ^ (t1 & tkc);
             }
-            uint32_t cw[4];
-#pragma unroll
-            for (int k = 0; k < 4; k++) cw[k] = sl[0][k] ^ sl[1][k];
-            store16(sCW + 16 * i, cw);
-            const uint32_t tLCW = tl[0] ^ tl[1] ^ (uint32_t)keep ^ 1u;
-            const uint32_t tRCW = tr[0] ^ tr[1] ^ (uint32_t)keep;
-            tlBits |= (T)((T)tLCW << i);
-            trBits |= (T)((T)tRCW << i);
-            const uint32_t m0 = 0u - t0, m1 = 0u - t1;
-#pragma unroll
-            for (int k = 0; k < 4; k++)
-            {
-                s0[k] = sk[0][k] ^ (cw[k] & m0);
-                s1[k] = sk[1][k] ^ (cw[k] & m1);
-            }
-            const uint32_t tk0 = keep ? tr[0] : tl[0], tk1 = keep ? tr[1] : tl[1], tkc = keep ? tRCW : tLCW;
-            t0 = tk0 ^ (t0 & tkc);
-            t1 = tk1 ^ (t1 & tkc);
-        }
-        // value correction of this level from the four value blocks (sign1
-        // is the party-1 control bit on entering the level)
-        for (int j0 = 0; j0 < g; j0 += SPC)
-        {
+            // this chunk's value corrections at level i (sign1 is the party-1
+            // control bit on entering the level)
             uint32_t w[2][NWC][4];
             T        v[SPC];
             convert_chunk_pair<OB, NWC>(c, w, vl1, vl0, direct, j0);
 #pragma unroll
             for (int r = 0; r < SPC; r++)
             {
-                const int j = j0 + r;
-                if (j < g)
-                {
-                    T x = (T)((T)(chunk_slot<T, OB, NWC>(w[0], r) - chunk_slot<T, OB, NWC>(w[1], r)) - last[j]);
-                    x   = (T)(x * sign1);
-                    if (keep)
-                    {
-                        const T b = beta ? beta[j] : (T)(j == 0 ? 1 : 0);
-                        x         = (T)(x + (T)(sign1 * b));
-                    }
-                    vcw[(size_t)i * g + j] = x;
-                    v[r]                   = x;
-                }
+                T x = (T)((T)(chunk_slot<T, OB, NWC>(w[0], r) - chunk_slot<T, OB, NWC>(w[1], r)) - last[r]);
+                x   = (T)(x * sign1);
+                if (keep) x = (T)(x + (T)(sign1 * bj[r]));
+                v[r] = x;
+                if (j0 + r < g) vcw[(size_t)i * g + j0 + r] = x;
             }
             convert_chunk_pair<OB, NWC>(c, w, vk1, vk0, direct, j0);
 #pragma unroll
             for (int r = 0; r < SPC; r++)
-            {
-                const int j = j0 + r;
-                if (j < g)
-                    last[j] = (T)(last[j] - chunk_slot<T, OB, NWC>(w[0], r) + chunk_slot<T, OB, NWC>(w[1], r) +
-                                  (T)(sign1 * v[r]));
-            }
+                last[r] = (T)(last[r] - chunk_slot<T, OB, NWC>(w[0], r) + chunk_slot<T, OB, NWC>(w[1], r) +
+                              (T)(sign1 * v[r]));
         }
-    }
-    tail[(size_t)n * g]     = tlBits;
-    tail[(size_t)n * g + 1] = trBits;
-    const T sign1 = t1 ? (T)-1 : (T)1;
-    for (int j0 = 0; j0 < g; j0 += SPC)
-    {
+        if (first)
+        {
+            tail[(size_t)n * g]     = tlBits;
+            tail[(size_t)n * g + 1] = trBits;
+        }
+        const T  sign1 = t1 ? (T)-1 : (T)1;
         uint32_t w[2][NWC][4];
         convert_chunk_pair<OB, NWC>(c, w, s1, s0, direct, j0);
 #pragma unroll
         for (int r = 0; r < SPC; r++)
-        {
-            const int j = j0 + r;
-            if (j < g)
-                last[j] = (T)((T)((T)(0 - last[j]) + chunk_slot<T, OB, NWC>(w[0], r) -
-                                  chunk_slot<T, OB, NWC>(w[1], r)) *
-                              sign1);
-        }
+            if (j0 + r < g)
+                lastCW[j0 + r] = (T)((T)((T)(0 - last[r]) + chunk_slot<T, OB, NWC>(w[0], r) -
+                                         chunk_slot<T, OB, NWC>(w[1], r)) *
+                                     sign1);
     }
 }
 

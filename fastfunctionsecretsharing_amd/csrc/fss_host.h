@@ -13,10 +13,10 @@ namespace fss {
 // Launches use one 1024-thread workgroup per CU: the AES tables take 128 KB
 // of the CU's 160 KB LDS, so a second workgroup would not fit anyway.
 constexpr int kBlock = 1024;
-// groupSize > 1 keyGen: 512 threads (two waves per SIMD with the one
-// table-holding workgroup per CU), so each lane has 256 registers for the
-// four value blocks and their conversion windows
-constexpr int kBlockGroupKeyGen = 512;
+// groupSize > 1 keyGen: 768 threads (three waves per SIMD with the one
+// table-holding workgroup per CU), 168 registers per lane; the few values
+// spilled are live only across the outer slot-chunk loop
+constexpr int kBlockGroupKeyGen = 768;
 
 int device_cu_count(); // cached per device
 
