@@ -97,9 +97,7 @@ __device__ __forceinline__ uint64_t dcf_walk_rt(const AesCtx &c, const uint8_t *
             {
                 const uint32_t xi = (uint32_t)(x >> (n - 1 - i)) & 1u;
                 uint32_t       blk[2][4];
-                blk[0][0] = 4u * xi, blk[0][1] = 0u, blk[0][2] = 4u * xi + 1u, blk[0][3] = 0u;
-                blk[1][0] = 4u * xi + 2u, blk[1][1] = 0u, blk[1][2] = 4u * xi + 3u, blk[1][3] = 0u;
-                aes_encrypt<2>(c, s, blk);
+                aes_encrypt_dcf_pair(c, s, xi, blk);
                 const uint32_t mk = 0u - t;
                 uint32_t       tn = blk[0][0] & 1u;
                 s[0]              = (blk[0][0] & ~1u) ^ (cw[k][0] & mk);

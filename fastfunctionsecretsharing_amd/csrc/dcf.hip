@@ -35,7 +35,7 @@ __global__ __launch_bounds__(kBlock) void dcf_keygen_g1_kernel(uint8_t *key, con
 }
 
 template <typename T, int OB>
-__global__ __launch_bounds__(kBlockGroupKeyGen) void dcf_keygen_kernel(uint8_t *key, const T *alpha, const T *beta,
+__global__ __launch_bounds__(group_keygen_block(sizeof(T), OB)) void dcf_keygen_kernel(uint8_t *key, const T *alpha, const T *beta,
                                                             const uint8_t *seed0, const uint8_t *seed1, int n, int m,
                                                             int g, size_t N)
 {
@@ -340,7 +340,7 @@ extern "C" FAST_FSS_API int FastFss_cuda_dcfKeyGen(void *key, size_t keyDataSize
     {
         const int ob = keygen_chunk_ob((int)m, es);
         FSS_DISPATCH(es, FSS_KEYGEN_OB_DISPATCH(T, ob,
-                                                (dcf_keygen_kernel<T, OBV><<<grid_for(N), kBlockGroupKeyGen, 0, st>>>(
+                                                (dcf_keygen_kernel<T, OBV><<<grid_for(N), group_keygen_block(sizeof(T), OBV), 0, st>>>(
                                                     (uint8_t *)key, (const T *)alpha, (const T *)b,
                                                     (const uint8_t *)seed0, (const uint8_t *)seed1, (int)n, (int)m,
                                                     (int)g, N))));

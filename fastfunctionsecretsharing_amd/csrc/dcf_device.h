@@ -46,9 +46,7 @@ __device__ __forceinline__ T dcf_eval_g1(const AesCtx &c, const uint8_t *sCW, co
             vcwn = tail[i + 1];
         }
         uint32_t blk[2][4];
-        blk[0][0] = 4u * xi, blk[0][1] = 0u, blk[0][2] = 4u * xi + 1u, blk[0][3] = 0u;
-        blk[1][0] = 4u * xi + 2u, blk[1][1] = 0u, blk[1][2] = 4u * xi + 3u, blk[1][3] = 0u;
-        aes_encrypt<2>(c, s, blk);
+        aes_encrypt_dcf_pair(c, s, xi, blk);
         const uint32_t m  = 0u - t;
         uint32_t       tn = blk[0][0] & 1u;
         s[0]              = (blk[0][0] & ~1u) ^ (cw[0] & m);
@@ -123,7 +121,10 @@ __device__ __forceinline__ void dcf_eval_g1_burst(const AesCtx &c, T out[E], con
                     blk[e][1][0] = 4u * xi[e] + 2u, blk[e][1][1] = 0u, blk[e][1][2] = 4u * xi[e] + 3u,
                     blk[e][1][3] = 0u;
                 }
-                aes_encrypt_multi<E, 2>(c, s, blk);
+                if constexpr (E == 1)
+                    aes_encrypt_dcf_pair(c, s[0], xi[0], blk[0]);
+                else
+                    aes_encrypt_multi<E, 2>(c, s, blk);
 #pragma unroll
                 for (int e = 0; e < E; e++)
                 {
@@ -186,9 +187,7 @@ __device__ __forceinline__ T dcf_eval_g1_pre(const AesCtx &c, const uint8_t *sCW
             {
                 const uint32_t xi = (uint32_t)(x >> (n - 1 - i)) & 1u;
                 uint32_t       blk[2][4];
-                blk[0][0] = 4u * xi, blk[0][1] = 0u, blk[0][2] = 4u * xi + 1u, blk[0][3] = 0u;
-                blk[1][0] = 4u * xi + 2u, blk[1][1] = 0u, blk[1][2] = 4u * xi + 3u, blk[1][3] = 0u;
-                aes_encrypt<2>(c, s, blk);
+                aes_encrypt_dcf_pair(c, s, xi, blk);
                 const uint32_t m  = 0u - t;
                 uint32_t       tn = blk[0][0] & 1u;
                 s[0]              = (blk[0][0] & ~1u) ^ (cw[k][0] & m);
@@ -228,9 +227,7 @@ __device__ __forceinline__ void dcf_eval_range(const AesCtx &c, T acc[R], const 
         uint32_t       cw[4];
         load16(cw, sCW + 16 * i);
         uint32_t blk[2][4];
-        blk[0][0] = 4u * xi, blk[0][1] = 0u, blk[0][2] = 4u * xi + 1u, blk[0][3] = 0u;
-        blk[1][0] = 4u * xi + 2u, blk[1][1] = 0u, blk[1][2] = 4u * xi + 3u, blk[1][3] = 0u;
-        aes_encrypt<2>(c, s, blk);
+        aes_encrypt_dcf_pair(c, s, xi, blk);
         const uint32_t mk = 0u - t;
         uint32_t       tn = blk[0][0] & 1u;
         s[0]              = (blk[0][0] & ~1u) ^ (cw[0] & mk);
@@ -289,9 +286,7 @@ __device__ __forceinline__ void dcf_eval_slots(const AesCtx &c, T acc[R], const 
         uint32_t       cw[4];
         load16(cw, sCW + 16 * i);
         uint32_t blk[2][4];
-        blk[0][0] = 4u * xi, blk[0][1] = 0u, blk[0][2] = 4u * xi + 1u, blk[0][3] = 0u;
-        blk[1][0] = 4u * xi + 2u, blk[1][1] = 0u, blk[1][2] = 4u * xi + 3u, blk[1][3] = 0u;
-        aes_encrypt<2>(c, s, blk);
+        aes_encrypt_dcf_pair(c, s, xi, blk);
         const uint32_t mk = 0u - t;
         uint32_t       tn = blk[0][0] & 1u;
         s[0]              = (blk[0][0] & ~1u) ^ (cw[0] & mk);
@@ -343,9 +338,7 @@ __device__ void dcf_eval_global(const AesCtx &c, T *out, const uint8_t *sCW, con
         uint32_t       cw[4];
         load16(cw, sCW + 16 * i);
         uint32_t blk[2][4];
-        blk[0][0] = 4u * xi, blk[0][1] = 0u, blk[0][2] = 4u * xi + 1u, blk[0][3] = 0u;
-        blk[1][0] = 4u * xi + 2u, blk[1][1] = 0u, blk[1][2] = 4u * xi + 3u, blk[1][3] = 0u;
-        aes_encrypt<2>(c, s, blk);
+        aes_encrypt_dcf_pair(c, s, xi, blk);
         const uint32_t mk = 0u - t;
         uint32_t       tn = blk[0][0] & 1u;
         s[0]              = (blk[0][0] & ~1u) ^ (cw[0] & mk);
@@ -403,17 +396,13 @@ __device__ void dcf_keygen_g1(const AesCtx &c, uint8_t *sCW, T *tail, T alpha, T
             const uint32_t keep = (uint32_t)(alpha >> (n - 1 - i)) & 1u;
             uint32_t       b[4][4];
             uint32_t       sk0[4], sl0[4];
-#pragma unroll
-            for (int q = 0; q < 4; q++) prg_pt(b[q], q);
-            aes_encrypt<4>(c, s0, b);
+            aes_encrypt_prg4(c, s0, b);
             const uint32_t tL0 = b[0][0] & 1u, tR0 = b[2][0] & 1u;
             b[0][0] &= ~1u, b[2][0] &= ~1u;
 #pragma unroll
             for (int w = 0; w < 4; w++) sk0[w] = keep ? b[2][w] : b[0][w], sl0[w] = keep ? b[0][w] : b[2][w];
             const T vl0 = (T)low_bytes(keep ? b[1] : b[3], ob), vk0 = (T)low_bytes(keep ? b[3] : b[1], ob);
-#pragma unroll
-            for (int q = 0; q < 4; q++) prg_pt(b[q], q);
-            aes_encrypt<4>(c, s1, b);
+            aes_encrypt_prg4(c, s1, b);
             const uint32_t tL1 = b[0][0] & 1u, tR1 = b[2][0] & 1u;
             b[0][0] &= ~1u, b[2][0] &= ~1u;
             const T vl1 = (T)low_bytes(keep ? b[1] : b[3], ob), vk1 = (T)low_bytes(keep ? b[3] : b[1], ob);
@@ -473,10 +462,8 @@ __device__ void dcf_keygen_one(const AesCtx &c, uint8_t *sCW, T *tail, T alpha, 
     for (int i = 0; i < n; i++)
     {
         uint32_t b0[4][4], b1[4][4];
-#pragma unroll
-        for (int q = 0; q < 4; q++) prg_pt(b0[q], q), prg_pt(b1[q], q);
-        aes_encrypt<4>(c, s0, b0);
-        aes_encrypt<4>(c, s1, b1);
+        aes_encrypt_prg4(c, s0, b0);
+        aes_encrypt_prg4(c, s1, b1);
         const uint32_t tL0 = b0[0][0] & 1u, tR0 = b0[2][0] & 1u, tL1 = b1[0][0] & 1u, tR1 = b1[2][0] & 1u;
         b0[0][0] &= ~1u, b0[2][0] &= ~1u, b1[0][0] &= ~1u, b1[2][0] &= ~1u;
         const int keep = (int)((alpha >> (n - 1 - i)) & 1);
@@ -625,9 +612,7 @@ __device__ void dcf_keygen_chunked(const AesCtx &c, uint8_t *sCW, T *tail, T alp
                 {
                     const uint32_t *sp = p ? s1 : s0;
                     uint32_t        b[4][4];
-#pragma unroll
-                    for (int q = 0; q < 4; q++) prg_pt(b[q], q);
-                    aes_encrypt<4>(c, sp, b);
+                    aes_encrypt_prg4(c, sp, b);
                     tl[p] = b[0][0] & 1u, tr[p] = b[2][0] & 1u;
                     b[0][0] &= ~1u, b[2][0] &= ~1u;
                     uint32_t *vl = p ? vl1 : vl0, *vk = p ? vk1 : vk0;

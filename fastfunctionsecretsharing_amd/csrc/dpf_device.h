@@ -19,9 +19,8 @@ __device__ __forceinline__ void tree_keygen_level(const AesCtx &c, uint32_t s0[4
                                                   uint32_t &tRCW)
 {
     uint32_t b0[2][4], b1[2][4];
-    prg_pt(b0[0], 0), prg_pt(b0[1], 1), prg_pt(b1[0], 0), prg_pt(b1[1], 1);
-    aes_encrypt<2>(c, s0, b0);
-    aes_encrypt<2>(c, s1, b1);
+    aes_encrypt_prg2(c, s0, b0);
+    aes_encrypt_prg2(c, s1, b1);
     const uint32_t tL0 = b0[0][0] & 1u, tR0 = b0[1][0] & 1u, tL1 = b1[0][0] & 1u, tR1 = b1[1][0] & 1u;
     b0[0][0] &= ~1u, b0[1][0] &= ~1u, b1[0][0] &= ~1u, b1[1][0] &= ~1u;
     // children chosen by masks: a selected array index would place b0/b1 in

@@ -351,10 +351,10 @@ __global__ __launch_bounds__(kBlock) void grotto_full_kernel(void *out, const ui
             while (depth < D)
             {
                 const int level = L + depth;
-                uint32_t  blk[2][4] = {{0u, 0u, 1u, 0u}, {2u, 0u, 3u, 0u}};
+                uint32_t  blk[2][4];
                 uint32_t  cw[4];
                 load16(cw, sc + 16 * level);
-                aes_encrypt<2>(c, s, blk);
+                aes_encrypt_prg2(c, s, blk);
                 const uint32_t mk = 0u - t;
                 const uint32_t tL = (blk[0][0] & 1u) ^ ((uint32_t)(tl >> level) & 1u & t);
                 const uint32_t tR = (blk[1][0] & 1u) ^ ((uint32_t)(tr >> level) & 1u & t);
@@ -476,10 +476,10 @@ __global__ __launch_bounds__(kBlock) void dpf_eval_all_dfs_kernel(T *out, const 
             while (depth < D)
             {
                 const int level     = L + depth;
-                uint32_t  blk[2][4] = {{0u, 0u, 1u, 0u}, {2u, 0u, 3u, 0u}};
+                uint32_t  blk[2][4];
                 uint32_t  cw[4];
                 load16(cw, sc + 16 * level);
-                aes_encrypt<2>(c, s, blk);
+                aes_encrypt_prg2(c, s, blk);
                 const uint32_t mk = 0u - t;
                 const uint32_t tl_ = (blk[0][0] & 1u) ^ ((uint32_t)(tL >> level) & 1u & t);
                 const uint32_t tr_ = (blk[1][0] & 1u) ^ ((uint32_t)(tR >> level) & 1u & t);

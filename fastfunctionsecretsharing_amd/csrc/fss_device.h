@@ -300,6 +300,69 @@ __device__ __forceinline__ void aes_encrypt_multi(const AesCtx &c, const uint32_
     }
 }
 
+// NB blocks under one key whose plaintexts are (p0[q], 0, p2[q], 0) as
+// little-endian u32 words with every p0[q], p2[q] < 256: the PRG counter
+// blocks of a tree level (DCF eval: (4x, 4x+1), (4x+2, 4x+3); keyGen:
+// (2q, 2q+1), q = 0..3).  After the whitening the blocks differ only in byte 0
+// of words 0 and 2.  Round 1 output word w reads byte 0 of word w and bytes 1-3
+// of the other words, so words 1 and 3 come out equal for every block and
+// words 0 and 2 differ only in their TE0 term: round 1 costs 14 shared table
+// lookups plus 2 per block instead of 16 per block.  Rounds 2-10 are the
+// ordinary ones.  Same output as aes_encrypt<NB> on those plaintexts.
+template <int NB>
+__device__ __forceinline__ void aes_encrypt_lowpt(const AesCtx &c, const uint32_t key[4], const uint32_t p0[NB],
+                                                  const uint32_t p2[NB], uint32_t st[NB][4])
+{
+    uint32_t       k[4] = {key[0], key[1], key[2], key[3]};
+    const uint32_t a0 = k[0], a1 = k[1], a2 = k[2], a3 = k[3]; // bytes 1-3 of the whitened words 0, 2
+    const uint32_t rcon[10] = {0x01, 0x02, 0x04, 0x08, 0x10, 0x20, 0x40, 0x80, 0x1b, 0x36};
+    aes_next_key(c, k, rcon[0]);
+    const uint32_t r0 = xor3(te<1>(c, a1, 1), te<2>(c, a2, 2), te<3>(c, a3, 3)) ^ k[0];
+    const uint32_t r1 = xor3(te<0>(c, a1, 0), te<1>(c, a2, 1), xor3(te<2>(c, a3, 2), te<3>(c, a0, 3), k[1]));
+    const uint32_t r2 = xor3(te<1>(c, a3, 1), te<2>(c, a0, 2), te<3>(c, a1, 3)) ^ k[2];
+    const uint32_t r3 = xor3(te<0>(c, a3, 0), te<1>(c, a0, 1), xor3(te<2>(c, a1, 2), te<3>(c, a2, 3), k[3]));
+#pragma unroll
+    for (int q = 0; q < NB; q++)
+    {
+        st[q][0] = r0 ^ te<0>(c, p0[q] ^ a0, 0);
+        st[q][1] = r1;
+        st[q][2] = r2 ^ te<0>(c, p2[q] ^ a2, 0);
+        st[q][3] = r3;
+    }
+#pragma unroll
+    for (int r = 1; r < 9; r++)
+    {
+        aes_next_key(c, k, rcon[r]);
+#pragma unroll
+        for (int q = 0; q < NB; q++) aes_round(c, st[q], k);
+    }
+    aes_next_key(c, k, rcon[9]);
+#pragma unroll
+    for (int q = 0; q < NB; q++) aes_last(c, st[q], k);
+}
+
+// The two blocks a DCF walk needs at one level (x = input bit).
+__device__ __forceinline__ void aes_encrypt_dcf_pair(const AesCtx &c, const uint32_t key[4], uint32_t xi,
+                                                     uint32_t st[2][4])
+{
+    const uint32_t p0[2] = {4u * xi, 4u * xi + 2u}, p2[2] = {4u * xi + 1u, 4u * xi + 3u};
+    aes_encrypt_lowpt<2>(c, key, p0, p2, st);
+}
+
+// The four counter blocks of a keyGen level: plaintexts (2q, 2q+1), q = 0..3.
+__device__ __forceinline__ void aes_encrypt_prg4(const AesCtx &c, const uint32_t key[4], uint32_t st[4][4])
+{
+    const uint32_t p0[4] = {0u, 2u, 4u, 6u}, p2[4] = {1u, 3u, 5u, 7u};
+    aes_encrypt_lowpt<4>(c, key, p0, p2, st);
+}
+
+// The two counter blocks of a DPF/Grotto keyGen level: (0, 1) and (2, 3).
+__device__ __forceinline__ void aes_encrypt_prg2(const AesCtx &c, const uint32_t key[4], uint32_t st[2][4])
+{
+    const uint32_t p0[2] = {0u, 2u}, p2[2] = {1u, 3u};
+    aes_encrypt_lowpt<2>(c, key, p0, p2, st);
+}
+
 // ---------------------------------------------------------------------------
 // PRG and value conversion
 // ---------------------------------------------------------------------------

@@ -13,10 +13,15 @@ namespace fss {
 // Launches use one 1024-thread workgroup per CU: the AES tables take 128 KB
 // of the CU's 160 KB LDS, so a second workgroup would not fit anyway.
 constexpr int kBlock = 1024;
-// groupSize > 1 keyGen: 768 threads (three waves per SIMD with the one
-// table-holding workgroup per CU), 168 registers per lane; the few values
-// spilled are live only across the outer slot-chunk loop
-constexpr int kBlockGroupKeyGen = 768;
+// groupSize > 1 keyGen workgroup size: 768 threads (three waves per SIMD
+// with the one table-holding workgroup per CU, 168 registers per lane; the
+// few values spilled are live only across the outer slot-chunk loop), 512
+// (256 registers) for 64-bit elements with conversions narrower than the
+// element, whose chunks hold 16 or more 64-bit slots
+constexpr int group_keygen_block(size_t es, int ob)
+{
+    return (es == 8 && ob > 0 && ob < 8) ? 512 : 768;
+}
 
 int device_cu_count(); // cached per device
 

@@ -134,7 +134,7 @@ __device__ void spline_shift(T *outc, const T *coef, T shift, int d, int m)
 }
 
 template <typename T, int OB>
-__global__ __launch_bounds__(kBlockGroupKeyGen) void spline_keygen_kernel(uint8_t *key, T *eOut, T *betaOut, const T *alpha,
+__global__ __launch_bounds__(group_keygen_block(sizeof(T), OB)) void spline_keygen_kernel(uint8_t *key, T *eOut, T *betaOut, const T *alpha,
                                                                const uint8_t *seed0, const uint8_t *seed1,
                                                                const T *coef, int d, const T *left, const T *right,
                                                                size_t I, int n, int m, size_t N)
@@ -476,7 +476,7 @@ extern "C" FAST_FSS_API int FastFss_cuda_dcfSplineKeyGen(
     const int ob = keygen_chunk_ob((int)m, es);
     FSS_DISPATCH(es, FSS_KEYGEN_OB_DISPATCH(
                          T, ob,
-                         (spline_keygen_kernel<T, OBV><<<grid_for(N), kBlockGroupKeyGen, 0, st>>>(
+                         (spline_keygen_kernel<T, OBV><<<grid_for(N), group_keygen_block(sizeof(T), OBV), 0, st>>>(
                              (uint8_t *)key, (T *)e, (T *)beta, (const T *)alpha, (const uint8_t *)seed0,
                              (const uint8_t *)seed1, (const T *)coefficients, (int)degree, (const T *)leftEndpoints,
                              (const T *)rightEndpoints, I, (int)n, (int)m, N))));
