@@ -341,6 +341,39 @@ __device__ __forceinline__ void aes_encrypt_lowpt(const AesCtx &c, const uint32_
     for (int q = 0; q < NB; q++) aes_last(c, st[q], k);
 }
 
+// aes_encrypt_lowpt for the conversion windows: plaintexts (0, 0, p2[q], 0)
+// with every p2[q] < 256, only the first `nb` (<= NB, wave-uniform) blocks.
+// The blocks differ only in byte 0 of word 2, so round 1 costs 15 shared
+// lookups plus 1 per block.
+template <int NB>
+__device__ __forceinline__ void aes_encrypt_win_upto(const AesCtx &c, const uint32_t key[4], const uint32_t p2[NB],
+                                                     uint32_t st[NB][4], int nb)
+{
+    uint32_t       k[4] = {key[0], key[1], key[2], key[3]};
+    const uint32_t a0 = k[0], a1 = k[1], a2 = k[2], a3 = k[3];
+    const uint32_t rcon[10] = {0x01, 0x02, 0x04, 0x08, 0x10, 0x20, 0x40, 0x80, 0x1b, 0x36};
+    aes_next_key(c, k, rcon[0]);
+    const uint32_t r0 = xor3(te<0>(c, a0, 0), te<1>(c, a1, 1), xor3(te<2>(c, a2, 2), te<3>(c, a3, 3), k[0]));
+    const uint32_t r1 = xor3(te<0>(c, a1, 0), te<1>(c, a2, 1), xor3(te<2>(c, a3, 2), te<3>(c, a0, 3), k[1]));
+    const uint32_t r2 = xor3(te<1>(c, a3, 1), te<2>(c, a0, 2), te<3>(c, a1, 3)) ^ k[2];
+    const uint32_t r3 = xor3(te<0>(c, a3, 0), te<1>(c, a0, 1), xor3(te<2>(c, a1, 2), te<3>(c, a2, 3), k[3]));
+#pragma unroll
+    for (int q = 0; q < NB; q++)
+        if (q < nb) st[q][0] = r0, st[q][1] = r1, st[q][2] = r2 ^ te<0>(c, p2[q] ^ a2, 0), st[q][3] = r3;
+#pragma unroll
+    for (int r = 1; r < 9; r++)
+    {
+        aes_next_key(c, k, rcon[r]);
+#pragma unroll
+        for (int q = 0; q < NB; q++)
+            if (q < nb) aes_round(c, st[q], k);
+    }
+    aes_next_key(c, k, rcon[9]);
+#pragma unroll
+    for (int q = 0; q < NB; q++)
+        if (q < nb) aes_last(c, st[q], k);
+}
+
 // The two blocks a DCF walk needs at one level (x = input bit).
 __device__ __forceinline__ void aes_encrypt_dcf_pair(const AesCtx &c, const uint32_t key[4], uint32_t xi,
                                                      uint32_t st[2][4])
@@ -491,10 +524,22 @@ struct SlotConv
         else
         {
             uint32_t blk[NW][4];
+            // (only for 4- and 8-byte slots: with narrower slots the extra
+            // shared terms push the spline kernel past its register budget)
+            if ((OB == 4 || OB == 8) && (16 * (w0 + NW - 1)) / OB < 256)
+            {
+                uint32_t p2[NW];
 #pragma unroll
-            for (int k = 0; k < NW; k++)
-                blk[k][0] = 0u, blk[k][1] = 0u, blk[k][2] = (uint32_t)((16 * (w0 + k)) / OB), blk[k][3] = 0u;
-            aes_encrypt_upto<NW>(c, v, blk, nw);
+                for (int k = 0; k < NW; k++) p2[k] = (uint32_t)((16 * (w0 + k)) / OB);
+                aes_encrypt_win_upto<NW>(c, v, p2, blk, nw);
+            }
+            else
+            {
+#pragma unroll
+                for (int k = 0; k < NW; k++)
+                    blk[k][0] = 0u, blk[k][1] = 0u, blk[k][2] = (uint32_t)((16 * (w0 + k)) / OB), blk[k][3] = 0u;
+                aes_encrypt_upto<NW>(c, v, blk, nw);
+            }
 #pragma unroll
             for (int k = 0; k < NW; k++)
 #pragma unroll
