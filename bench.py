@@ -38,6 +38,11 @@ BYTES_PER_EVAL_N32 = 652 + 16 + 4 + 4
 # HBM bytes per launch of the eval kernel at 2^24 elements, from the two
 # calibrated rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) recorded here
 TRAFFIC_PROFILE = REPO / "profiles" / "dcf_eval_queue_pre_n32_2p24_traffic.md"
+# LDS table lookups per eval of the same kernel (rocprofv3 SQ_INSTS_LDS x 64 /
+# evals, recorded here): the pipe that binds the T-table AES
+TRACE_PROFILE = REPO / "profiles" / "dcf_eval_queue_pre_n32_2p24_kernel_trace.md"
+# MI355X LDS: 128 B/clk/CU = 32 four-byte lookups per clock per CU
+LDS_PEAK_GLOOKUPS = 256 * 32 * 2.4e9 / 1e9
 
 
 def profiled_traffic(log2n: int):
@@ -45,6 +50,15 @@ def profiled_traffic(log2n: int):
         return None
     for line in TRAFFIC_PROFILE.read_text().splitlines():
         if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":", 1)[1])
+    return None
+
+
+def profiled_lds_per_eval(log2n: int):
+    if log2n != 24 or not TRACE_PROFILE.exists():
+        return None
+    for line in TRACE_PROFILE.read_text().splitlines():
+        if line.startswith("lds_instructions_per_eval:"):
             return int(line.split(":", 1)[1])
     return None
 
@@ -424,6 +438,8 @@ def main():
 
     ops = n * OPS_PER_LEVEL * N
     achieved_tops = ops / (kernel_ms * 1e-3) / 1e12
+    lds_per_eval = profiled_lds_per_eval(a.log2n)
+    lds_g = lds_per_eval * N / (kernel_ms * 1e-3) / 1e9 if lds_per_eval else None
     hbm_gbs = BYTES_PER_EVAL_N32 * N / (kernel_ms * 1e-3) / 1e9
 
     res = {
@@ -456,6 +472,14 @@ def main():
             "ops_per_eval": n * OPS_PER_LEVEL,
             "hbm_achieved_gbs": hbm_gbs,
             "hbm_frac": hbm_gbs / HBM_PEAK_GBS,
+            "lds_lookups_per_eval": lds_per_eval,
+            "lds_achieved_glookups": lds_g,
+            "lds_peak_glookups": LDS_PEAK_GLOOKUPS,
+            "lds_frac": (lds_g / LDS_PEAK_GLOOKUPS) if lds_g else None,
+            "model_note": "achieved counts the fixed 760-op/level VALU model (32 ops per T-table round, every "
+                          "round of both blocks); the kernel shares round 1 between the two blocks and issues "
+                          "fewer VALU instructions (rocprof), so frac can exceed 1. The pipe that binds is LDS: "
+                          "lds_frac uses the kernel's measured lookups per eval",
         },
     }
     del key, out
