@@ -858,7 +858,7 @@ extern "C" FAST_FSS_API int FastFss_cuda_grottoIntervalLutEval(
                          (const uint8_t *)seed, partyId, (const T *)leftEndpoints, (const T *)rightEndpoints,
                          (const T *)lookUpTable, lutNum, I, (int)n, N));
     int rc = launch_status();
-    hipFreeAsync(bits, st);
+    (void)hipFreeAsync(bits, st);
     return rc;
 }
 
