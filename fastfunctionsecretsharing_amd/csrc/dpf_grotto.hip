@@ -126,7 +126,7 @@ struct GrottoKeyPtrs
     const uint8_t  *tlr;
 };
 
-__device__ __forceinline__ GrottoKeyPtrs grotto_key(const uint8_t *key, int n, size_t es, size_t N)
+__device__ __forceinline__ GrottoKeyPtrs grotto_key(const uint8_t *key, int n, size_t N)
 {
     GrottoKeyPtrs k;
     k.sCW    = key;
@@ -158,7 +158,7 @@ __global__ __launch_bounds__(kBlock) void grotto_eval_kernel(T *out, const T *x,
 {
     // mode 0: [x < alpha] share, 1: [x <= alpha] share, 2: [x == alpha] share
     FSS_KERNEL_PROLOGUE
-    const GrottoKeyPtrs k   = grotto_key(key, n, sizeof(T), N);
+    const GrottoKeyPtrs k   = grotto_key(key, n, N);
     const T            *tlr = (const T *)k.tlr;
     FSS_GRID_LOOP(e, N)
     {
@@ -200,7 +200,7 @@ __global__ __launch_bounds__(kBlock) void grotto_mic_kernel(T *out, const T *x, 
                                                             size_t N)
 {
     FSS_KERNEL_PROLOGUE
-    const GrottoKeyPtrs k   = grotto_key(key, n, sizeof(T), N);
+    const GrottoKeyPtrs k   = grotto_key(key, n, N);
     const T            *tlr = (const T *)k.tlr;
     FSS_GRID_LOOP(e, N)
     {
@@ -220,7 +220,7 @@ __global__ __launch_bounds__(kBlock) void grotto_lut_kernel(T *outE, T *outT, T 
                                                             int n, size_t N)
 {
     FSS_KERNEL_PROLOGUE
-    const GrottoKeyPtrs k   = grotto_key(key, n, sizeof(T), N);
+    const GrottoKeyPtrs k   = grotto_key(key, n, N);
     const T            *tlr = (const T *)k.tlr;
     FSS_GRID_LOOP(e, N)
     {
@@ -312,7 +312,7 @@ __global__ __launch_bounds__(kBlock) void grotto_full_kernel(void *out, const ui
                                                              size_t N)
 {
     FSS_KERNEL_PROLOGUE
-    const GrottoKeyPtrs k     = grotto_key(key, n, sizeof(T), N);
+    const GrottoKeyPtrs k     = grotto_key(key, n, N);
     const T            *tlr   = (const T *)k.tlr;
     const size_t        lanes = N << L;
     const int           lv    = n - 6; // tree levels above the leaf words
