@@ -1,8 +1,6 @@
 """The autotuner picks valid launch shapes on the device, applies them, and the
 tuned shapes leave every output unchanged (tiles and eval variants only move
 work between lanes)."""
-import ctypes
-
 import pytest
 
 pytestmark = pytest.mark.gpu

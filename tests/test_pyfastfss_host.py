@@ -2,7 +2,6 @@
 reference package (lib/torch_extension/pyFastFss/__init__.py:5-70), size
 queries equal to the oracle's, the reference's argument checks and reshaping,
 and a loud error (not a CPU fallback) for tensors that are not on a GPU."""
-import numpy as np
 import pytest
 import torch
 
