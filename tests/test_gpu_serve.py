@@ -87,3 +87,51 @@ def test_submit_validation(env):
     assert lib.FastFss_hip_serverRelease(srv, tk.value) == -1
     assert lib.FastFss_hip_serverRelease(srv, tk.value + 1000) == -1
     lib.FastFss_hip_serverDestroy(srv)
+
+
+def test_concurrent_submitters(env):
+    """Four host threads submit into one server at once; every request still
+    equals its alone result, and each ticket is released after its wait."""
+    import ctypes
+    import threading
+
+    torch, _lib, W, templates = env
+    lib = _lib.load()
+    rng = np.random.default_rng(17)
+    picks = [[int(p) for p in rng.integers(0, len(templates), size=10)] for _ in range(4)]
+    srv = ctypes.c_void_p()
+    assert lib.FastFss_hip_serverCreate(ctypes.byref(srv), 1 << 34, 8, 4, 1) == 0
+    outs = [[None] * 10 for _ in range(4)]
+    tickets = [[0] * 10 for _ in range(4)]
+    reqs = [[None] * 10 for _ in range(4)]  # kept alive until the server is done
+    errors = []
+
+    def submitter(k):
+        try:
+            for i, p in enumerate(picks[k]):
+                t = templates[p]
+                outs[k][i] = torch.empty(t.out_elems(), dtype=W.torch_dtype(t.es), device="cuda:0")
+                reqs[k][i] = W.serve_request(t, outs[k][i])
+                tk = ctypes.c_uint64()
+                assert lib.FastFss_hip_serverSubmit(srv, ctypes.byref(reqs[k][i]), ctypes.byref(tk)) == 0
+                tickets[k][i] = tk.value
+        except Exception as ex:  # surfaced below on the test thread
+            errors.append(ex)
+
+    torch.cuda.synchronize()  # output allocations are complete before any request runs
+    threads = [threading.Thread(target=submitter, args=(k,)) for k in range(4)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    assert not errors, errors
+    ids = [tk for row in tickets for tk in row]
+    assert len(set(ids)) == len(ids)
+    for tk in ids:
+        assert lib.FastFss_hip_serverWait(srv, tk, None) == 0
+        assert lib.FastFss_hip_serverRelease(srv, tk) == 0
+    lib.FastFss_hip_serverDestroy(srv)
+    for k in range(4):
+        ref = _alone(torch, W, templates, picks[k])
+        for i in range(10):
+            assert torch.equal(outs[k][i], ref[i]), (k, i)
