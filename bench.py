@@ -124,17 +124,24 @@ def cpu_baseline(log2n: int):
     }
 
 
-def ragged_ops(groups):
-    """Algorithmic VALU ops of one pass over a ragged batch: DCF n x 760, DPF n x 440
-    plus one more AES level for the value conversion when bitWidthOut > 16."""
+# per-unit costs: (key schedule, block) in VALU-model ops and in AES table lookups
+VALU_COST = (10 * 12, 10 * 32)
+LDS_COST = (10 * 4, 10 * 16)
+
+
+def ragged_ops(groups, cost=VALU_COST):
+    """Algorithmic work of one pass over a ragged batch: DCF n x (schedule + 2 blocks),
+    DPF n x (schedule + block) plus one more level for the value conversion when
+    bitWidthOut > 16.  `cost` = (key schedule, block): VALU-model ops or lookups."""
     from fastfunctionsecretsharing_amd import _lib
 
+    ks, blk = cost
     ops = 0
     for g in groups:
         if g.op == _lib.BATCH_DCF_EVAL:
-            ops += g.N * g.n * OPS_PER_LEVEL
+            ops += g.N * g.n * (ks + 2 * blk)
         else:
-            ops += g.N * (g.n + (1 if g.n > 16 else 0)) * DPF_OPS_PER_LEVEL
+            ops += g.N * (g.n + (1 if g.n > 16 else 0)) * (ks + blk)
     return ops
 
 
@@ -199,6 +206,7 @@ def ragged_leg(groups_n: int, reps: int = 5):
         "plain_device_ms": plain_dev_ms,
         "batched_equals_plain": same,
         "batched_valu_frac": ragged_ops(groups) / (batch_dev_ms * 1e-3) / 1e12 / VALU_PEAK_TOPS,
+        "batched_lds_frac": ragged_ops(groups, LDS_COST) / (batch_dev_ms * 1e-3) / 1e9 / LDS_PEAK_GLOOKUPS,
         "plain_valu_frac": ragged_ops(groups) / (plain_dev_ms * 1e-3) / 1e12 / VALU_PEAK_TOPS,
         "groups_per_width": widths,
         "autotuned_tiles": {"%s/es%d" % ("dcf" if op == 0 else "dpf", es): t for (op, es), t in sorted(tiles.items())},
@@ -246,10 +254,11 @@ def dpf_leg(log2n: int, reps: int = 10):
     del holder
     return {"workload": "plain DPF eval, bitWidthIn=32, bitWidthOut=32, groupSize=1, 2^%d elements, party 0" % log2n,
             "evals_per_s": N / wall, "kernel_ms": dev_ms, "ops_per_eval": ops, "valu_tops": achieved,
-            "valu_frac": achieved / VALU_PEAK_TOPS}
+            "valu_frac": achieved / VALU_PEAK_TOPS,
+            "lds_frac": (n + 1) * sum(LDS_COST) * N / (dev_ms * 1e-3) / 1e9 / LDS_PEAK_GLOOKUPS}
 
 
-def spline_ops(w):
+def spline_ops(w, cost=VALU_COST):
     """Algorithmic VALU ops of one spline element: the range walks of the
     interval sweep (one per interval end, plus one per interval start that does
     not follow its predecessor), each n levels of (key schedule + 2 blocks) for
@@ -258,7 +267,8 @@ def spline_ops(w):
     left = [int(v) for v in w.left.cpu().tolist()]
     right = [int(v) for v in w.right.cpu().tolist()]
     mask = (1 << w.n) - 1
-    C, OB, ks, blk = w.d + 1, (w.m + 7) // 8, 120, 320
+    C, OB = w.d + 1, (w.m + 7) // 8
+    ks, blk = cost
     g = w.I * C
 
     def walk(j0, cnt):
@@ -296,7 +306,9 @@ def piecewise_leg(reps: int = 3):
     ops = (I + 1) * w.n * OPS_PER_LEVEL  # contiguous intervals share endpoints: I + 1 DCF evals per element
     res["mic"] = {"workload": "DCF-MIC, 16 contiguous intervals, bitWidthIn=24, bitWidthOut=32, 2^22 elements",
                   "elements_per_s": N / wall, "interval_evals_per_s": N * I / wall, "kernel_ms": dev_ms,
-                  "valu_frac": ops * N / (dev_ms * 1e-3) / 1e12 / VALU_PEAK_TOPS}
+                  "valu_frac": ops * N / (dev_ms * 1e-3) / 1e12 / VALU_PEAK_TOPS,
+                  "lds_frac": (I + 1) * w.n * (LDS_COST[0] + 2 * LDS_COST[1]) * N / (dev_ms * 1e-3) / 1e9
+                  / LDS_PEAK_GLOOKUPS}
     del out
     cache = W.spline_cache(w)
     out = torch.empty(N, dtype=torch.int32, device="cuda:0")
@@ -306,7 +318,8 @@ def piecewise_leg(reps: int = 3):
     res["spline"] = {"workload": "DCF-spline degree 2, 16 contiguous intervals, bitWidthIn=24, bitWidthOut=32, "
                                  "2^22 elements, int32 coefficients",
                      "elements_per_s": N / wall, "kernel_ms": dev_ms, "key_bytes": w.spl_ks,
-                     "ops_per_element": ops // N, "valu_frac": ops / (dev_ms * 1e-3) / 1e12 / VALU_PEAK_TOPS}
+                     "ops_per_element": ops // N, "valu_frac": ops / (dev_ms * 1e-3) / 1e12 / VALU_PEAK_TOPS,
+                     "lds_frac": N * spline_ops(w, LDS_COST) / (dev_ms * 1e-3) / 1e9 / LDS_PEAK_GLOOKUPS}
     del w, out, cache
     torch.cuda.empty_cache()
     gs = W.grotto_keys()
@@ -320,6 +333,8 @@ def piecewise_leg(reps: int = 3):
         wall, dev_ms = _timed(stream, lambda: W.grotto_full(gs, 0, _lib.GROTTO_FULL_LT, packed, out, sp), reps * 3)
         full[name] = {"points_per_s": pts / wall, "kernel_ms": dev_ms,
                       "valu_frac": ops / (dev_ms * 1e-3) / 1e12 / VALU_PEAK_TOPS,
+                      "lds_frac": gs.N * ((1 << lv) - 1) * (LDS_COST[0] + 2 * LDS_COST[1]) / (dev_ms * 1e-3) / 1e9
+                      / LDS_PEAK_GLOOKUPS,
                       "out_gbs": out.numel() * out.element_size() / (dev_ms * 1e-3) / 1e9}
         del out
     res["grotto_full_domain"] = full
