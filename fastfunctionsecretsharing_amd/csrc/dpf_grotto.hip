@@ -342,8 +342,13 @@ __global__ __launch_bounds__(kBlock) void grotto_full_kernel(void *out, const ui
             P ^= bit & (prev ^ t);
         }
 
-        uint32_t stS[kFullMaxDepth][4];
-        uint32_t stTP[kFullMaxDepth]; // t | parity << 1 of the pending right child
+        // pending right children of depths 0 .. D-2; the last level's right
+        // child is a leaf, kept in (pr2, pr3, prtp) and emitted right after
+        // its sibling, so it never goes through the stack
+        uint32_t stS[kFullMaxDepth - 1][4];
+        uint32_t stTP[kFullMaxDepth - 1]; // t | parity << 1 of the pending right child
+        uint32_t pr2 = 0, pr3 = 0, prtp = 0;
+        const int pair = D > 0 ? 2 : 1; // leaves emitted per descent
         const uint32_t leaves = 1u << D;
         int            depth  = 0;
         for (uint32_t leaf = 0;;)
@@ -361,16 +366,25 @@ __global__ __launch_bounds__(kBlock) void grotto_full_kernel(void *out, const ui
                 const uint32_t rs0 = (blk[1][0] & ~1u) ^ (cw[0] & mk), rs1 = blk[1][1] ^ (cw[1] & mk);
                 const uint32_t rs2 = blk[1][2] ^ (cw[2] & mk), rs3 = blk[1][3] ^ (cw[3] & mk);
                 const uint32_t rtp = tR | ((P ^ t ^ tR) << 1);
+                if (depth == D - 1)
+                {
+                    pr2  = rs2;
+                    pr3  = rs3;
+                    prtp = rtp;
+                }
+                else
+                {
 #pragma unroll
-                for (int d = 0; d < kFullMaxDepth; d++)
-                    if (d == depth)
-                    {
-                        stS[d][0] = rs0;
-                        stS[d][1] = rs1;
-                        stS[d][2] = rs2;
-                        stS[d][3] = rs3;
-                        stTP[d]   = rtp;
-                    }
+                    for (int d = 0; d < kFullMaxDepth - 1; d++)
+                        if (d == depth)
+                        {
+                            stS[d][0] = rs0;
+                            stS[d][1] = rs1;
+                            stS[d][2] = rs2;
+                            stS[d][3] = rs3;
+                            stTP[d]   = rtp;
+                        }
+                }
                 s[0] = (blk[0][0] & ~1u) ^ (cw[0] & mk);
                 s[1] = blk[0][1] ^ (cw[1] & mk);
                 s[2] = blk[0][2] ^ (cw[2] & mk);
@@ -379,6 +393,16 @@ __global__ __launch_bounds__(kBlock) void grotto_full_kernel(void *out, const ui
                 depth++;
             }
 
+#pragma unroll 1
+            for (int k = 0; k < pair; k++, leaf++)
+            {
+            if (k == 1)
+            {
+                s[2] = pr2;
+                s[3] = pr3;
+                t    = prtp & 1u;
+                P    = prtp >> 1;
+            }
             const uint64_t u = ((uint64_t)s[2] | ((uint64_t)s[3] << 32)) ^ (t ? lastCW : 0ull);
             uint64_t       bits;
             if (mode == 2)
@@ -416,14 +440,16 @@ __global__ __launch_bounds__(kBlock) void grotto_full_kernel(void *out, const ui
                 }
             }
 
-            if (++leaf == leaves) break;
-            // the next leaf's path leaves the current one at depth D-1-ctz(leaf):
-            // resume from the right child pushed there
+            }
+            if (leaf == leaves) break;
+            // the next pair's path leaves the current one at depth
+            // D-1-ctz(leaf) <= D-2: resume from the right child pushed there
             const int d0 = D - 1 - (int)__builtin_ctz(leaf);
 #pragma unroll
-            for (int d = 0; d < kFullMaxDepth; d++)
+            for (int d = 0; d < kFullMaxDepth - 1; d++)
                 if (d == d0)
                 {
+                    FSS_UNIFORM_CASE;
                     s[0] = stS[d][0];
                     s[1] = stS[d][1];
                     s[2] = stS[d][2];
@@ -467,8 +493,13 @@ __global__ __launch_bounds__(kBlock) void dpf_eval_all_dfs_kernel(T *out, const 
         for (int i = 0; i < L; i++)
             tree_eval_step<T>(c, s, t, (uint32_t)(r >> (L - 1 - i)) & 1u, sc + 16 * i, tL, tR, i);
 
-        uint32_t       stS[kFullMaxDepth][4];
-        uint32_t       stT[kFullMaxDepth];
+        // pending right children of depths 0 .. D-2; the last level's right
+        // child is a leaf, kept in (pr, prt) and converted right after its
+        // sibling, so it never goes through the stack
+        uint32_t       stS[kFullMaxDepth - 1][4];
+        uint32_t       stT[kFullMaxDepth - 1];
+        uint32_t       pr[4] = {0u, 0u, 0u, 0u}, prt = 0;
+        const int      pair = D > 0 ? 2 : 1; // leaves converted per descent
         const uint32_t leaves = 1u << D;
         int            depth  = 0;
         for (uint32_t leaf = 0;;)
@@ -485,16 +516,27 @@ __global__ __launch_bounds__(kBlock) void dpf_eval_all_dfs_kernel(T *out, const 
                 const uint32_t tr_ = (blk[1][0] & 1u) ^ ((uint32_t)(tR >> level) & 1u & t);
                 const uint32_t r0 = (blk[1][0] & ~1u) ^ (cw[0] & mk), r1 = blk[1][1] ^ (cw[1] & mk);
                 const uint32_t r2 = blk[1][2] ^ (cw[2] & mk), r3 = blk[1][3] ^ (cw[3] & mk);
+                if (depth == D - 1)
+                {
+                    pr[0] = r0;
+                    pr[1] = r1;
+                    pr[2] = r2;
+                    pr[3] = r3;
+                    prt   = tr_;
+                }
+                else
+                {
 #pragma unroll
-                for (int d = 0; d < kFullMaxDepth; d++)
-                    if (d == depth)
-                    {
-                        stS[d][0] = r0;
-                        stS[d][1] = r1;
-                        stS[d][2] = r2;
-                        stS[d][3] = r3;
-                        stT[d]    = tr_;
-                    }
+                    for (int d = 0; d < kFullMaxDepth - 1; d++)
+                        if (d == depth)
+                        {
+                            stS[d][0] = r0;
+                            stS[d][1] = r1;
+                            stS[d][2] = r2;
+                            stS[d][3] = r3;
+                            stT[d]    = tr_;
+                        }
+                }
                 s[0] = (blk[0][0] & ~1u) ^ (cw[0] & mk);
                 s[1] = blk[0][1] ^ (cw[1] & mk);
                 s[2] = blk[0][2] ^ (cw[2] & mk);
@@ -503,6 +545,17 @@ __global__ __launch_bounds__(kBlock) void dpf_eval_all_dfs_kernel(T *out, const 
                 depth++;
             }
 
+#pragma unroll 1
+            for (int k = 0; k < pair; k++, leaf++)
+            {
+            if (k == 1)
+            {
+                s[0] = pr[0];
+                s[1] = pr[1];
+                s[2] = pr[2];
+                s[3] = pr[3];
+                t    = prt;
+            }
             const size_t j   = (size_t)((r << D) | leaf);
             T           *dst = oe + ((xe - j) & (size - 1)) * (size_t)g;
             if (g == 1)
@@ -530,12 +583,15 @@ __global__ __launch_bounds__(kBlock) void dpf_eval_all_dfs_kernel(T *out, const 
                 }
             }
 
-            if (++leaf == leaves) break;
+            }
+            if (leaf == leaves) break;
+            // resume from the right child pushed at depth D-1-ctz(leaf) <= D-2
             const int d0 = D - 1 - (int)__builtin_ctz(leaf);
 #pragma unroll
-            for (int d = 0; d < kFullMaxDepth; d++)
+            for (int d = 0; d < kFullMaxDepth - 1; d++)
                 if (d == d0)
                 {
+                    FSS_UNIFORM_CASE;
                     s[0] = stS[d][0];
                     s[1] = stS[d][1];
                     s[2] = stS[d][2];

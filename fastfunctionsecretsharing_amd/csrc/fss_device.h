@@ -28,6 +28,13 @@
 #include <stddef.h>
 #include <stdint.h>
 
+// Opens a conditional block whose condition is wave-uniform and keeps it a
+// branch: an empty volatile asm cannot be speculated, so the compiler does
+// not flatten the block into per-lane selects of every value it assigns.
+#ifndef FSS_UNIFORM_CASE
+#define FSS_UNIFORM_CASE asm volatile("")
+#endif
+
 namespace fss {
 
 // ---------------------------------------------------------------------------
