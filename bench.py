@@ -378,7 +378,7 @@ def serving_leg(qps: float, requests: int, sweep_qps=(200, 800, "burst")):
                     "mean_ms": float(lat_ms.mean()), "wall_s": wall, "max_inflight": int(st.maxInflightSeen),
                     "config": {"hbm_budget_bytes": cfg[0], "max_inflight": cfg[1], "streams": cfg[2],
                                "launch": "one workgroup per 1024 elements" if cfg[3]
-                               else "persistent, one workgroup per CU"}})
+                               else "the per-call entry points' launch shape"}})
         row["outputs_equal"] = all(torch.equal(a, b) for a, b in zip(out["batched"], out["plain"]))
         return row
 

@@ -60,8 +60,8 @@ __global__ __launch_bounds__(kBlock) void dcf_eval_g1_kernel(T *out, const T *x,
                                                              const uint8_t *seed, int party, int n, int outBytes,
                                                              size_t N)
 {
-    __shared__ uint32_t lds[kLdsTableWords];
-    aes_fill_lds(lds);
+    __shared__ __align__(16) uint32_t lds[kLdsTableWords];
+    aes_fill_lds_staged(lds);
     __syncthreads();
     const AesCtx   c     = aes_ctx(lds);
     const size_t   sw    = 16 * (size_t)n;
@@ -83,8 +83,8 @@ __global__ __launch_bounds__(kBlock) void dcf_eval_burst_kernel(T *out, const T 
                                                                 const uint8_t *seed, int party, int n, int outBytes,
                                                                 size_t N)
 {
-    __shared__ uint32_t lds[kLdsTableWords];
-    aes_fill_lds(lds);
+    __shared__ __align__(16) uint32_t lds[kLdsTableWords];
+    aes_fill_lds_staged(lds);
     __syncthreads();
     const AesCtx c     = aes_ctx(lds);
     const size_t sw    = 16 * (size_t)n;
@@ -119,8 +119,8 @@ __global__ __launch_bounds__(kBlock) void dcf_eval_group_kernel(T *out, const T 
                                                                 const uint8_t *seed, int party, int n, int m, int g,
                                                                 size_t N)
 {
-    __shared__ uint32_t lds[kLdsTableWords];
-    aes_fill_lds(lds);
+    __shared__ __align__(16) uint32_t lds[kLdsTableWords];
+    aes_fill_lds_staged(lds);
     __syncthreads();
     const AesCtx c     = aes_ctx(lds);
     const size_t sw    = 16 * (size_t)n;
@@ -243,8 +243,8 @@ int dcf_eval_variant(size_t es, size_t n, size_t N)
 }
 
 template <typename T>
-int launch_dcf_eval_g1(int v, int grid, hipStream_t st, T *out, const T *x, const uint8_t *key, const uint8_t *seed,
-                       int party, int n, int outBytes, size_t N)
+int launch_dcf_eval_g1(int v, int grid, int block, hipStream_t st, T *out, const T *x, const uint8_t *key,
+                       const uint8_t *seed, int party, int n, int outBytes, size_t N)
 {
     if (v == 5 || v == 6)
     {
@@ -252,20 +252,20 @@ int launch_dcf_eval_g1(int v, int grid, hipStream_t st, T *out, const T *x, cons
         if (!counter) return FAST_FSS_RUNTIME_ERROR;
         if (v == 6 && sizeof(T) <= 4)
             dcf_eval_queue_kernel<T, 8, (sizeof(T) <= 4)>
-                <<<grid, kBlock, 0, st>>>(out, x, key, seed, party, n, outBytes, N, counter);
+                <<<grid, block, 0, st>>>(out, x, key, seed, party, n, outBytes, N, counter);
         else
             dcf_eval_queue_kernel<T, 8, false>
-                <<<grid, kBlock, 0, st>>>(out, x, key, seed, party, n, outBytes, N, counter);
+                <<<grid, block, 0, st>>>(out, x, key, seed, party, n, outBytes, N, counter);
         queue_counter_release(st, counter);
         return FAST_FSS_SUCCESS;
     }
     switch (v)
     {
-        case 0: dcf_eval_g1_kernel<T><<<grid, kBlock, 0, st>>>(out, x, key, seed, party, n, outBytes, N); break;
-        case 2: dcf_eval_burst_kernel<T, 8, 1><<<grid, kBlock, 0, st>>>(out, x, key, seed, party, n, outBytes, N); break;
-        case 3: dcf_eval_burst_kernel<T, 4, 2><<<grid, kBlock, 0, st>>>(out, x, key, seed, party, n, outBytes, N); break;
-        case 4: dcf_eval_burst_kernel<T, 2, 2><<<grid, kBlock, 0, st>>>(out, x, key, seed, party, n, outBytes, N); break;
-        default: dcf_eval_burst_kernel<T, 4, 1><<<grid, kBlock, 0, st>>>(out, x, key, seed, party, n, outBytes, N); break;
+        case 0: dcf_eval_g1_kernel<T><<<grid, block, 0, st>>>(out, x, key, seed, party, n, outBytes, N); break;
+        case 2: dcf_eval_burst_kernel<T, 8, 1><<<grid, block, 0, st>>>(out, x, key, seed, party, n, outBytes, N); break;
+        case 3: dcf_eval_burst_kernel<T, 4, 2><<<grid, block, 0, st>>>(out, x, key, seed, party, n, outBytes, N); break;
+        case 4: dcf_eval_burst_kernel<T, 2, 2><<<grid, block, 0, st>>>(out, x, key, seed, party, n, outBytes, N); break;
+        default: dcf_eval_burst_kernel<T, 4, 1><<<grid, block, 0, st>>>(out, x, key, seed, party, n, outBytes, N); break;
     }
     return FAST_FSS_SUCCESS;
 }
@@ -273,12 +273,12 @@ int launch_dcf_eval_g1(int v, int grid, hipStream_t st, T *out, const T *x, cons
 // Serving launches choose their own grid shape, so they keep the static split
 // (variants 5 and 6 fall back to variant 2 there).
 int launch_dcf_eval_g1_rt(size_t es, int grid, hipStream_t st, void *out, const void *x, const void *key,
-                          const void *seed, int party, int n, int m, size_t N)
+                          const void *seed, int party, int n, int m, size_t N, int block)
 {
     int v = dcf_eval_variant(es, (size_t)n, N);
     if (v == 5 || v == 6) v = 2;
     int rc = FAST_FSS_SUCCESS;
-    FSS_DISPATCH(es, rc = launch_dcf_eval_g1<T>(v, grid, st, (T *)out, (const T *)x, (const uint8_t *)key,
+    FSS_DISPATCH(es, rc = launch_dcf_eval_g1<T>(v, grid, block, st, (T *)out, (const T *)x, (const uint8_t *)key,
                                                 (const uint8_t *)seed, party, n, (m + 7) / 8, N));
     return rc != FAST_FSS_SUCCESS ? rc : launch_status();
 }
@@ -331,7 +331,8 @@ extern "C" FAST_FSS_API int FastFss_cuda_dcfKeyGen(void *key, size_t keyDataSize
     {
         uint32_t *counter = queue_counter_acquire(st);
         if (!counter) return FAST_FSS_RUNTIME_ERROR;
-        FSS_DISPATCH(es, dcf_keygen_g1_kernel<T><<<grid_for(N), kBlock, 0, st>>>(
+        const LaunchShape ls = shape_for(N);
+        FSS_DISPATCH(es, dcf_keygen_g1_kernel<T><<<ls.grid, ls.block, 0, st>>>(
                              (uint8_t *)key, (const T *)alpha, (const T *)b, (const uint8_t *)seed0,
                              (const uint8_t *)seed1, (int)n, (int)m, N, counter));
         queue_counter_release(st, counter);
@@ -369,19 +370,20 @@ extern "C" FAST_FSS_API int FastFss_cuda_dcfEval(void *sharedOut, size_t sharedO
     if (cacheDataSize != 0 && cacheDataSize != dcf_cache_size(n, g, es, N)) return FAST_FSS_INVALID_CACHE_DATA_SIZE_ERROR;
     if (N == 0) return FAST_FSS_SUCCESS;
     if (!aligned(key, 8) || !aligned(seed, 8)) return FAST_FSS_RUNTIME_ERROR;
-    hipStream_t st = resolve_stream(cudaStreamPtr);
+    hipStream_t       st = resolve_stream(cudaStreamPtr);
+    const LaunchShape ls = shape_for(N);
     if (g == 1)
     {
         const int v  = dcf_eval_variant(es, n, N);
         int       rc = FAST_FSS_SUCCESS;
-        FSS_DISPATCH(es, rc = launch_dcf_eval_g1<T>(v, grid_for(N), st, (T *)sharedOut, (const T *)maskedX,
+        FSS_DISPATCH(es, rc = launch_dcf_eval_g1<T>(v, ls.grid, ls.block, st, (T *)sharedOut, (const T *)maskedX,
                                                     (const uint8_t *)key, (const uint8_t *)seed, partyId, (int)n,
                                                     (int)((m + 7) / 8), N));
         if (rc != FAST_FSS_SUCCESS) return rc;
     }
     else
     {
-        FSS_DISPATCH(es, dcf_eval_group_kernel<T><<<grid_for(N), kBlock, 0, st>>>(
+        FSS_DISPATCH(es, dcf_eval_group_kernel<T><<<ls.grid, ls.block, 0, st>>>(
                              (T *)sharedOut, (const T *)maskedX, (const uint8_t *)key, (const uint8_t *)seed, partyId,
                              (int)n, (int)m, (int)g, N));
     }

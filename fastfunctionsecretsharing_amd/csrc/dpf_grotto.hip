@@ -21,6 +21,13 @@ namespace fss {
     __syncthreads();                        \
     const AesCtx c = aes_ctx(lds);
 
+// the same for kernels that often run in small workgroups (aes_fill_lds_staged)
+#define FSS_KERNEL_PROLOGUE_STAGED                    \
+    __shared__ __align__(16) uint32_t lds[kLdsTableWords]; \
+    aes_fill_lds_staged(lds);                         \
+    __syncthreads();                                  \
+    const AesCtx c = aes_ctx(lds);
+
 #define FSS_GRID_LOOP(i, count)                                                             \
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < (count); \
          i += (size_t)gridDim.x * blockDim.x)
@@ -47,7 +54,7 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void dpf_eval_kernel(T *out, const T *x, const uint8_t *key, const uint8_t *seed,
                                                           int party, int n, int m, int g, size_t N)
 {
-    FSS_KERNEL_PROLOGUE
+    FSS_KERNEL_PROLOGUE_STAGED
     const T *tails = (const T *)(key + N * 16 * (size_t)n);
     FSS_GRID_LOOP(e, N)
     {
@@ -62,7 +69,7 @@ template <typename T, int CH>
 __global__ __launch_bounds__(kBlock) void dpf_eval_burst_kernel(T *out, const T *x, const uint8_t *key,
                                                                 const uint8_t *seed, int party, int n, int m, size_t N)
 {
-    FSS_KERNEL_PROLOGUE
+    FSS_KERNEL_PROLOGUE_STAGED
     const T *tails = (const T *)(key + N * 16 * (size_t)n);
     FSS_GRID_LOOP(e, N)
     {
@@ -105,7 +112,7 @@ __global__ __launch_bounds__(kBlock) void dpf_eval_multi_kernel(T *out, const T 
                                                                 const uint8_t *seed, int party, const T *points,
                                                                 size_t P, int n, int m, int g, size_t N)
 {
-    FSS_KERNEL_PROLOGUE
+    FSS_KERNEL_PROLOGUE_STAGED
     const T *tails = (const T *)(key + N * 16 * (size_t)n);
     FSS_GRID_LOOP(w, N * P)
     {
@@ -157,7 +164,7 @@ __global__ __launch_bounds__(kBlock) void grotto_eval_kernel(T *out, const T *x,
                                                              size_t N)
 {
     // mode 0: [x < alpha] share, 1: [x <= alpha] share, 2: [x == alpha] share
-    FSS_KERNEL_PROLOGUE
+    FSS_KERNEL_PROLOGUE_STAGED
     const GrottoKeyPtrs k   = grotto_key(key, n, N);
     const T            *tlr = (const T *)k.tlr;
     FSS_GRID_LOOP(e, N)
@@ -199,7 +206,7 @@ __global__ __launch_bounds__(kBlock) void grotto_mic_kernel(T *out, const T *x, 
                                                             int party, const T *left, const T *right, size_t I, int n,
                                                             size_t N)
 {
-    FSS_KERNEL_PROLOGUE
+    FSS_KERNEL_PROLOGUE_STAGED
     const GrottoKeyPtrs k   = grotto_key(key, n, N);
     const T            *tlr = (const T *)k.tlr;
     FSS_GRID_LOOP(e, N)
@@ -219,7 +226,7 @@ __global__ __launch_bounds__(kBlock) void grotto_lut_kernel(T *outE, T *outT, T 
                                                             const T *right, const T *lut, size_t lutNum, size_t I,
                                                             int n, size_t N)
 {
-    FSS_KERNEL_PROLOGUE
+    FSS_KERNEL_PROLOGUE_STAGED
     const GrottoKeyPtrs k   = grotto_key(key, n, N);
     const T            *tlr = (const T *)k.tlr;
     FSS_GRID_LOOP(e, N)
@@ -619,9 +626,9 @@ inline void full_domain_split(int lv, size_t N, int &L, int &D)
 }
 
 int launch_dpf_eval_g1_rt(size_t es, int grid, hipStream_t st, void *out, const void *x, const void *key,
-                          const void *seed, int party, int n, int m, size_t N)
+                          const void *seed, int party, int n, int m, size_t N, int block)
 {
-    FSS_DISPATCH(es, dpf_eval_burst_kernel<T, 8><<<grid, kBlock, 0, st>>>((T *)out, (const T *)x,
+    FSS_DISPATCH(es, dpf_eval_burst_kernel<T, 8><<<grid, block, 0, st>>>((T *)out, (const T *)x,
                                                                            (const uint8_t *)key,
                                                                            (const uint8_t *)seed, party, n, m, N));
     return launch_status();
@@ -660,7 +667,7 @@ extern "C" FAST_FSS_API int FastFss_cuda_dpfKeyGen(void *key, size_t keyDataSize
     const void *b  = betaDataSize ? beta : nullptr;
     uint32_t *counter = queue_counter_acquire(st);
     if (!counter) return FAST_FSS_RUNTIME_ERROR;
-    FSS_DISPATCH(es, dpf_keygen_kernel<T><<<grid_for(N), kBlock, 0, st>>>((uint8_t *)key, (const T *)alpha,
+    FSS_DISPATCH(es, dpf_keygen_kernel<T><<<FSS_SHAPE(N), 0, st>>>((uint8_t *)key, (const T *)alpha,
                                                                            (const T *)b, (const uint8_t *)seed0,
                                                                            (const uint8_t *)seed1, (int)n, (int)m,
                                                                            (int)g, N, counter));
@@ -701,20 +708,20 @@ extern "C" FAST_FSS_API int FastFss_cuda_dpfEval(void *sharedOut, size_t sharedO
     {
         uint32_t *counter = queue_counter_acquire(st);
         if (!counter) return FAST_FSS_RUNTIME_ERROR;
-        FSS_DISPATCH(es, dpf_eval_queue_kernel<T, 8><<<grid_for(N), kBlock, 0, st>>>(
+        FSS_DISPATCH(es, dpf_eval_queue_kernel<T, 8><<<FSS_SHAPE(N), 0, st>>>(
                              (T *)sharedOut, (const T *)maskedX, (const uint8_t *)key, (const uint8_t *)seed,
                              partyId, (int)n, (int)m, N, counter));
         queue_counter_release(st, counter);
     }
     else if (g == 1)
     {
-        FSS_DISPATCH(es, dpf_eval_burst_kernel<T, 8><<<grid_for(N), kBlock, 0, st>>>(
+        FSS_DISPATCH(es, dpf_eval_burst_kernel<T, 8><<<FSS_SHAPE(N), 0, st>>>(
                              (T *)sharedOut, (const T *)maskedX, (const uint8_t *)key, (const uint8_t *)seed,
                              partyId, (int)n, (int)m, N));
     }
     else
     {
-        FSS_DISPATCH(es, dpf_eval_kernel<T><<<grid_for(N), kBlock, 0, st>>>((T *)sharedOut, (const T *)maskedX,
+        FSS_DISPATCH(es, dpf_eval_kernel<T><<<FSS_SHAPE(N), 0, st>>>((T *)sharedOut, (const T *)maskedX,
                                                                              (const uint8_t *)key,
                                                                              (const uint8_t *)seed, partyId, (int)n,
                                                                              (int)m, (int)g, N));
@@ -744,7 +751,7 @@ extern "C" FAST_FSS_API int FastFss_cuda_dpfEvalAll(void *sharedOut, size_t shar
     hipStream_t st = resolve_stream(cudaStreamPtr);
     int         L, D;
     full_domain_split((int)n, N, L, D);
-    FSS_DISPATCH(es, dpf_eval_all_dfs_kernel<T><<<grid_for(N << L), kBlock, 0, st>>>(
+    FSS_DISPATCH(es, dpf_eval_all_dfs_kernel<T><<<FSS_SHAPE(N << L), 0, st>>>(
                          (T *)sharedOut, (const T *)maskedX, (const uint8_t *)key, (const uint8_t *)seed, partyId,
                          (int)n, (int)m, (int)g, L, D, N));
     return launch_status();
@@ -772,7 +779,7 @@ extern "C" FAST_FSS_API int FastFss_cuda_dpfEvalMulti(void *sharedOut, size_t sh
     if (N == 0 || P == 0) return FAST_FSS_SUCCESS;
     if (!aligned(key, 8) || !aligned(seed, 8)) return FAST_FSS_RUNTIME_ERROR;
     hipStream_t st = resolve_stream(cudaStreamPtr);
-    FSS_DISPATCH(es, dpf_eval_multi_kernel<T><<<grid_for(N * P), kBlock, 0, st>>>(
+    FSS_DISPATCH(es, dpf_eval_multi_kernel<T><<<FSS_SHAPE(N * P), 0, st>>>(
                          (T *)sharedOut, (const T *)maskedX, (const uint8_t *)key, (const uint8_t *)seed, partyId,
                          (const T *)point, P, (int)n, (int)m, (int)g, N));
     return launch_status();
@@ -794,7 +801,7 @@ extern "C" FAST_FSS_API int FastFss_cuda_grottoKeyGen(void *key, size_t keyDataS
     if (N == 0) return FAST_FSS_SUCCESS;
     if (!aligned(key, 8) || !aligned(seed0, 8) || !aligned(seed1, 8)) return FAST_FSS_RUNTIME_ERROR;
     hipStream_t st = resolve_stream(cudaStreamPtr);
-    FSS_DISPATCH(es, grotto_keygen_kernel<T><<<grid_for(N), kBlock, 0, st>>>(
+    FSS_DISPATCH(es, grotto_keygen_kernel<T><<<FSS_SHAPE(N), 0, st>>>(
                          (uint8_t *)key, (const T *)alpha, (const uint8_t *)seed0, (const uint8_t *)seed1, (int)n, N));
     return launch_status();
 }
@@ -821,7 +828,7 @@ static int grotto_eval_launch(void *out, size_t outSize, const void *x, size_t x
     if (N == 0) return FAST_FSS_SUCCESS;
     if (!aligned(key, 8) || !aligned(seed, 8)) return FAST_FSS_RUNTIME_ERROR;
     hipStream_t st = resolve_stream(sp);
-    FSS_DISPATCH(es, grotto_eval_kernel<T><<<grid_for(N), kBlock, 0, st>>>(
+    FSS_DISPATCH(es, grotto_eval_kernel<T><<<FSS_SHAPE(N), 0, st>>>(
                          (T *)out, (const T *)x, (const uint8_t *)key, (const uint8_t *)seed, party, (int)n, mode, N));
     return launch_status();
 }
@@ -876,7 +883,7 @@ extern "C" FAST_FSS_API int FastFss_cuda_grottoMICEval(void *sharedBooleanOut, s
     if (N == 0 || I == 0) return FAST_FSS_SUCCESS;
     if (!aligned(key, 8) || !aligned(seed, 8)) return FAST_FSS_RUNTIME_ERROR;
     hipStream_t st = resolve_stream(cudaStreamPtr);
-    FSS_DISPATCH(es, grotto_mic_kernel<T><<<grid_for(N), kBlock, 0, st>>>(
+    FSS_DISPATCH(es, grotto_mic_kernel<T><<<FSS_SHAPE(N), 0, st>>>(
                          (T *)sharedBooleanOut, (const T *)maskedX, (const uint8_t *)key, (const uint8_t *)seed,
                          partyId, (const T *)leftEndpoints, (const T *)rightEndpoints, I, (int)n, N));
     return launch_status();
@@ -909,7 +916,7 @@ extern "C" FAST_FSS_API int FastFss_cuda_grottoIntervalLutEval(
     hipStream_t st   = resolve_stream(cudaStreamPtr);
     void       *bits = nullptr;
     if (hipMallocAsync(&bits, N * I * es, st) != hipSuccess) return FAST_FSS_RUNTIME_ERROR;
-    FSS_DISPATCH(es, grotto_lut_kernel<T><<<grid_for(N), kBlock, 0, st>>>(
+    FSS_DISPATCH(es, grotto_lut_kernel<T><<<FSS_SHAPE(N), 0, st>>>(
                          (T *)sharedOutE, (T *)sharedOutT, (T *)bits, (const T *)maskedX, (const uint8_t *)key,
                          (const uint8_t *)seed, partyId, (const T *)leftEndpoints, (const T *)rightEndpoints,
                          (const T *)lookUpTable, lutNum, I, (int)n, N));
@@ -942,7 +949,7 @@ extern "C" FAST_FSS_API int FastFss_hip_grottoEvalFullDomain(void *sharedOut, si
     int L, D;
     full_domain_split((int)n - 6, N, L, D);
     hipStream_t st = resolve_stream(hipStreamPtr);
-    FSS_DISPATCH(es, grotto_full_kernel<T><<<grid_for(N << L), kBlock, 0, st>>>(
+    FSS_DISPATCH(es, grotto_full_kernel<T><<<FSS_SHAPE(N << L), 0, st>>>(
                          sharedOut, (const uint8_t *)key, (const uint8_t *)seed, partyId, mode, packed, (int)n, L, D,
                          N));
     return launch_status();

@@ -41,6 +41,33 @@ inline int grid_for(size_t work, int wavesPerItemBlock = 1)
     return (int)blocks;
 }
 
+// Workgroup count and size for `work` items of one lane each.  Work that
+// fills every CU gets the persistent shape, one maxBlock-lane workgroup per
+// CU.  Smaller work is spread over all CUs in smaller workgroups (whole
+// waves) rather than packed into full workgroups on a few CUs: a small call
+// then takes about the latency of one element's walk instead of waiting on
+// the LDS throughput of the few CUs it would occupy.
+struct LaunchShape
+{
+    int grid, block;
+};
+
+inline LaunchShape shape_for(size_t work, int maxBlock = kBlock)
+{
+    const size_t cus = (size_t)device_cu_count();
+    size_t       b   = (work + cus - 1) / cus;
+    b                = (b + 63) / 64 * 64;
+    if (b < 64) b = 64;
+    if (b > (size_t)maxBlock) b = (size_t)maxBlock;
+    size_t g = (work + b - 1) / b;
+    if (g > cus) g = cus;
+    if (g == 0) g = 1;
+    return {(int)g, (int)b};
+}
+
+// kernel<<<FSS_SHAPE(work), 0, stream>>>: the grid and block of shape_for(work)
+#define FSS_SHAPE(work) ::fss::shape_for(work).grid, ::fss::shape_for(work).block
+
 inline int launch_status()
 {
     return hipPeekAtLastError() == hipSuccess ? FAST_FSS_SUCCESS : FAST_FSS_RUNTIME_ERROR;

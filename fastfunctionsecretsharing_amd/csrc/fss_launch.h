@@ -1,13 +1,16 @@
 // Internal launchers shared between translation units (not part of the C ABI).
-// `grid` = number of 1024-thread workgroups; the kernels loop grid-stride, so
-// any grid >= 1 is correct.  The public entry points pass the persistent
-// one-per-CU grid; the serving runtime passes one workgroup per 1024
-// elements so that concurrent requests' workgroups interleave on the CUs.
+// `grid` workgroups of `block` lanes (a multiple of 64, at most 1024); the
+// kernels loop grid-stride, so any shape is correct.  The public entry points
+// pass shape_for(work); the serving runtime passes one 1024-lane workgroup
+// per 1024 elements so that concurrent requests' workgroups interleave on the
+// CUs.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 #include <stdint.h>
+
+#include "fss_host.h"
 
 namespace fss {
 
@@ -17,19 +20,20 @@ uint32_t *queue_counter_acquire(hipStream_t st);
 void      queue_counter_release(hipStream_t st, uint32_t *p);
 
 int launch_dcf_eval_g1_rt(size_t es, int grid, hipStream_t st, void *out, const void *x, const void *key,
-                          const void *seed, int party, int n, int m, size_t N);
+                          const void *seed, int party, int n, int m, size_t N, int block = kBlock);
 
 int launch_dpf_eval_g1_rt(size_t es, int grid, hipStream_t st, void *out, const void *x, const void *key,
-                          const void *seed, int party, int n, int m, size_t N);
+                          const void *seed, int party, int n, int m, size_t N, int block = kBlock);
 
 // counter: zeroed work-queue counter, or nullptr for the static grid split
 int launch_mic_eval_rt(size_t es, int grid, hipStream_t st, void *out, const void *x, const void *key, const void *z,
                        const void *seed, int party, const void *left, const void *right, size_t I, int n, int m,
-                       size_t N, uint32_t *counter);
+                       size_t N, uint32_t *counter, int block = kBlock);
 
 // degree <= 7 (the path that needs no per-element workspace)
 int launch_spline_eval_rt(size_t es, int grid, hipStream_t st, void *out, const void *x, const void *key,
                           const void *e, const void *beta, const void *seed, int party, const void *left,
-                          const void *right, size_t I, int degree, int n, int m, size_t N, uint32_t *counter);
+                          const void *right, size_t I, int degree, int n, int m, size_t N, uint32_t *counter,
+                          int block = kBlock);
 
 } // namespace fss

@@ -313,10 +313,10 @@ static size_t mic_blocks_per_elem(size_t N, size_t I)
 
 int launch_mic_eval_rt(size_t es, int grid, hipStream_t st, void *out, const void *x, const void *key, const void *z,
                        const void *seed, int party, const void *left, const void *right, size_t I, int n, int m,
-                       size_t N, uint32_t *counter)
+                       size_t N, uint32_t *counter, int block)
 {
     const size_t bpe = mic_blocks_per_elem(N, I);
-    FSS_DISPATCH(es, mic_eval_kernel<T><<<grid, kBlock, 0, st>>>((T *)out, (const T *)x, (const uint8_t *)key,
+    FSS_DISPATCH(es, mic_eval_kernel<T><<<grid, block, 0, st>>>((T *)out, (const T *)x, (const uint8_t *)key,
                                                                  (const T *)z, (const uint8_t *)seed, party,
                                                                  (const T *)left, (const T *)right, I, bpe, n,
                                                                  (m + 7) / 8, N, counter));
@@ -324,11 +324,11 @@ int launch_mic_eval_rt(size_t es, int grid, hipStream_t st, void *out, const voi
 }
 
 template <typename T, int OB>
-static void spline_launch_ob(int grid, hipStream_t st, void *out, const void *x, const void *key, const void *e,
+static void spline_launch_ob(int grid, int block, hipStream_t st, void *out, const void *x, const void *key, const void *e,
                              const void *beta, const void *seed, int party, const void *left, const void *right,
                              size_t I, int degree, int n, int m, size_t N, uint32_t *counter)
 {
-    spline_eval_kernel<T, OB><<<grid, kBlock, 0, st>>>((T *)out, (const T *)x, (const uint8_t *)key, (const T *)e,
+    spline_eval_kernel<T, OB><<<grid, block, 0, st>>>((T *)out, (const T *)x, (const uint8_t *)key, (const T *)e,
                                                        (const T *)beta, (const uint8_t *)seed, party, (const T *)left,
                                                        (const T *)right, I, degree, n, m, N, counter);
 }
@@ -336,7 +336,7 @@ static void spline_launch_ob(int grid, hipStream_t st, void *out, const void *x,
 // degree <= 7: the slot-conversion kernel for OB = ceil(m/8) when a linked
 // walk's 2(degree+1) slots fit 32 bytes, else the generic conversion.
 template <typename T>
-static void spline_launch(int grid, hipStream_t st, void *out, const void *x, const void *key, const void *e,
+static void spline_launch(int grid, int block, hipStream_t st, void *out, const void *x, const void *key, const void *e,
                           const void *beta, const void *seed, int party, const void *left, const void *right, size_t I,
                           int degree, int n, int m, size_t N, uint32_t *counter)
 {
@@ -347,7 +347,7 @@ static void spline_launch(int grid, hipStream_t st, void *out, const void *x, co
     case K:                                                                                                         \
         if constexpr (K <= (int)sizeof(T))                                                                          \
         {                                                                                                           \
-            spline_launch_ob<T, K>(grid, st, out, x, key, e, beta, seed, party, left, right, I, degree, n, m, N,   \
+            spline_launch_ob<T, K>(grid, block, st, out, x, key, e, beta, seed, party, left, right, I, degree, n, m, N,   \
                                    counter);                                                                        \
             return;                                                                                                 \
         }                                                                                                           \
@@ -365,15 +365,16 @@ static void spline_launch(int grid, hipStream_t st, void *out, const void *x, co
         default: break;
     }
 #undef FSS_SPLINE_OB
-    spline_launch_ob<T, 0>(grid, st, out, x, key, e, beta, seed, party, left, right, I, degree, n, m, N, counter);
+    spline_launch_ob<T, 0>(grid, block, st, out, x, key, e, beta, seed, party, left, right, I, degree, n, m, N, counter);
 }
 
 int launch_spline_eval_rt(size_t es, int grid, hipStream_t st, void *out, const void *x, const void *key,
                           const void *e, const void *beta, const void *seed, int party, const void *left,
-                          const void *right, size_t I, int degree, int n, int m, size_t N, uint32_t *counter)
+                          const void *right, size_t I, int degree, int n, int m, size_t N, uint32_t *counter,
+                          int block)
 {
     if (degree > 7) return FAST_FSS_RUNTIME_ERROR;
-    FSS_DISPATCH(es, spline_launch<T>(grid, st, out, x, key, e, beta, seed, party, left, right, I, degree, n, m, N,
+    FSS_DISPATCH(es, spline_launch<T>(grid, block, st, out, x, key, e, beta, seed, party, left, right, I, degree, n, m, N,
                                       counter));
     return launch_status();
 }
@@ -406,7 +407,7 @@ extern "C" FAST_FSS_API int FastFss_cuda_dcfMICKeyGen(void *key, size_t keyDataS
     hipStream_t st = resolve_stream(cudaStreamPtr);
     uint32_t *counter = queue_counter_acquire(st);
     if (!counter) return FAST_FSS_RUNTIME_ERROR;
-    FSS_DISPATCH(es, mic_keygen_kernel<T><<<grid_for(N), kBlock, 0, st>>>(
+    FSS_DISPATCH(es, mic_keygen_kernel<T><<<FSS_SHAPE(N), 0, st>>>(
                          (uint8_t *)key, (T *)z, (const T *)alpha, (const uint8_t *)seed0, (const uint8_t *)seed1,
                          (const T *)leftEndpoints, (const T *)rightEndpoints, I, (int)n, (int)m, N, counter));
     queue_counter_release(st, counter);
@@ -443,8 +444,9 @@ extern "C" FAST_FSS_API int FastFss_cuda_dcfMICEval(void *sharedOut, size_t shar
     const size_t bpe     = mic_blocks_per_elem(N, I);
     uint32_t    *counter = queue_counter_acquire(st);
     if (!counter) return FAST_FSS_RUNTIME_ERROR;
-    const int rc = launch_mic_eval_rt(es, grid_for(N * bpe), st, sharedOut, maskedX, key, sharedZ, seed, partyId,
-                                      leftEndpoints, rightEndpoints, I, (int)n, (int)m, N, counter);
+    const LaunchShape ls = shape_for(N * bpe);
+    const int rc = launch_mic_eval_rt(es, ls.grid, st, sharedOut, maskedX, key, sharedZ, seed, partyId, leftEndpoints,
+                                      rightEndpoints, I, (int)n, (int)m, N, counter, ls.block);
     queue_counter_release(st, counter);
     return rc;
 }
@@ -514,15 +516,16 @@ extern "C" FAST_FSS_API int FastFss_cuda_dcfSplineEval(
     {
         uint32_t *counter = queue_counter_acquire(st);
         if (!counter) return FAST_FSS_RUNTIME_ERROR;
-        const int rc = launch_spline_eval_rt(es, grid_for(N), st, sharedOut, maskedX, key, sharedE, sharedBeta, seed,
+        const LaunchShape ls = shape_for(N);
+        const int rc = launch_spline_eval_rt(es, ls.grid, st, sharedOut, maskedX, key, sharedE, sharedBeta, seed,
                                              partyId, leftEndpoints, rightEndpoints, I, (int)degree, (int)n, (int)m,
-                                             N, counter);
+                                             N, counter, ls.block);
         queue_counter_release(st, counter);
         return rc;
     }
     else
     {
-        FSS_DISPATCH(es, spline_eval_wide_kernel<T><<<grid_for(N), kBlock, 0, st>>>(
+        FSS_DISPATCH(es, spline_eval_wide_kernel<T><<<FSS_SHAPE(N), 0, st>>>(
                              (T *)sharedOut, (const T *)maskedX, (const uint8_t *)key, (const T *)sharedE,
                              (const T *)sharedBeta, (const uint8_t *)seed, partyId, (const T *)leftEndpoints,
                              (const T *)rightEndpoints, I, (int)degree, (int)n, (int)m, N, (uint8_t *)cache));

@@ -134,7 +134,7 @@ extern "C" FAST_FSS_API int FastFss_cuda_otttKeyGen(void *key, size_t keyDataSiz
     if (alphaDataSize != N * es) return FAST_FSS_INVALID_ALPHA_DATA_SIZE_ERROR;
     if (N == 0) return FAST_FSS_SUCCESS;
     hipStream_t st = resolve_stream(cudaStreamPtr);
-    FSS_DISPATCH(es, ottt_keygen_kernel<T><<<grid_for(N), kBlock, 0, st>>>((uint8_t *)key, (const T *)alpha, (int)n,
+    FSS_DISPATCH(es, ottt_keygen_kernel<T><<<FSS_SHAPE(N), 0, st>>>((uint8_t *)key, (const T *)alpha, (int)n,
                                                                             N));
     return launch_status();
 }
@@ -219,7 +219,7 @@ extern "C" FAST_FSS_API int FastFss_cuda_prngGen(void *prng, void *deviceDst, si
     hipStream_t  st    = resolve_stream(cudaStreamPtr);
     if (nblk)
     {
-        prng_gen_kernel<<<grid_for((nblk + 3) / 4), kBlock, 0, st>>>((const PrngState *)prng, (uint8_t *)deviceDst,
+        prng_gen_kernel<<<FSS_SHAPE((nblk + 3) / 4), 0, st>>>((const PrngState *)prng, (uint8_t *)deviceDst,
                                                                      bytes);
         if (hipPeekAtLastError() != hipSuccess) return FAST_FSS_RUNTIME_ERROR;
     }

@@ -99,26 +99,28 @@ static int launch(Server *s, ServeItem *it, hipStream_t st)
     if (r.elementNum == 0) return FAST_FSS_SUCCESS;
     size_t work = r.elementNum;
     if (r.op == FAST_FSS_SERVE_MIC_EVAL) work *= r.intervalNum > 4 ? 4 : r.intervalNum;
-    int grid = s->launchMode == 1 ? (int)((work + kBlock - 1) / kBlock) : grid_for(work);
-    if (grid > 65535) grid = 65535;
-    if (grid < 1) grid = 1;
+    // mode 1: one full workgroup per 1024 elements; mode 0: the per-call shape
+    LaunchShape ls = s->launchMode == 1 ? LaunchShape{(int)((work + kBlock - 1) / kBlock), kBlock} : shape_for(work);
+    if (ls.grid > 65535) ls.grid = 65535;
+    if (ls.grid < 1) ls.grid = 1;
+    const int grid = ls.grid;
     const int n = (int)r.bitWidthIn, m = (int)r.bitWidthOut;
     switch (r.op)
     {
         case FAST_FSS_SERVE_DCF_EVAL:
             return launch_dcf_eval_g1_rt(r.elementSize, grid, st, r.sharedOut, r.maskedX, r.key, r.seed, r.partyId, n,
-                                         m, r.elementNum);
+                                         m, r.elementNum, ls.block);
         case FAST_FSS_SERVE_DPF_EVAL:
             return launch_dpf_eval_g1_rt(r.elementSize, grid, st, r.sharedOut, r.maskedX, r.key, r.seed, r.partyId, n,
-                                         m, r.elementNum);
+                                         m, r.elementNum, ls.block);
         case FAST_FSS_SERVE_MIC_EVAL:
             return launch_mic_eval_rt(r.elementSize, grid, st, r.sharedOut, r.maskedX, r.key, r.sharedZ, r.seed,
                                       r.partyId, r.leftEndpoints, r.rightEndpoints, r.intervalNum, n, m,
-                                      r.elementNum, nullptr);
+                                      r.elementNum, nullptr, ls.block);
         default:
             return launch_spline_eval_rt(r.elementSize, grid, st, r.sharedOut, r.maskedX, r.key, r.sharedZ,
                                          r.sharedBeta, r.seed, r.partyId, r.leftEndpoints, r.rightEndpoints,
-                                         r.intervalNum, (int)r.degree, n, m, r.elementNum, nullptr);
+                                         r.intervalNum, (int)r.degree, n, m, r.elementNum, nullptr, ls.block);
     }
 }
 

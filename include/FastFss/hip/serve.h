@@ -68,8 +68,10 @@ typedef struct FastFssServeStats
     double   busySeconds; /* wall time with at least one request in flight */
 } FastFssServeStats;
 
-/* launchMode: 0 = one persistent workgroup per CU (the per-call entry points'
- * shape), 1 = one workgroup per 1024 elements (interleaves across requests). */
+/* launchMode: 0 = the per-call entry points' shape (one 1024-lane workgroup
+ * per CU once the request fills every CU, smaller workgroups spread over all
+ * CUs below that), 1 = one 1024-lane workgroup per 1024 elements (interleaves
+ * across requests). */
 FAST_FSS_API int FastFss_hip_serverCreate(void **server, size_t hbmBudgetBytes, int maxInflight, int numStreams,
                                           int launchMode);
 
