@@ -148,14 +148,10 @@ __global__ __launch_bounds__(kBlock) void dcf_eval_group_kernel(T *out, const T 
 // until the tiles run out, so waves that run fast take more tiles and the
 // launch ends when the work does, not when the slowest fixed share does.
 template <typename T, int CH, bool PRE>
-__global__ __launch_bounds__(kBlock) void dcf_eval_queue_kernel(T *out, const T *x, const uint8_t *key,
-                                                                const uint8_t *seed, int party, int n, int outBytes,
-                                                                size_t N, uint32_t *counter)
+__device__ __forceinline__ void dcf_eval_queue_body(const AesCtx &c, T *out, const T *x, const uint8_t *key,
+                                                    const uint8_t *seed, int party, int n, int outBytes, size_t N,
+                                                    uint32_t *counter)
 {
-    __shared__ uint32_t lds[kLdsTableWords];
-    aes_fill_lds(lds);
-    __syncthreads();
-    const AesCtx   c     = aes_ctx(lds);
     const size_t   sw    = 16 * (size_t)n;
     const T       *tails = (const T *)(key + N * sw);
     const size_t   ts    = (size_t)n + 3;
@@ -189,6 +185,31 @@ __global__ __launch_bounds__(kBlock) void dcf_eval_queue_kernel(T *out, const T 
             }
         }
     }
+}
+
+template <typename T, int CH, bool PRE>
+__global__ __launch_bounds__(kBlock) void dcf_eval_queue_kernel(T *out, const T *x, const uint8_t *key,
+                                                                const uint8_t *seed, int party, int n, int outBytes,
+                                                                size_t N, uint32_t *counter)
+{
+    __shared__ uint32_t lds[kLdsTableWords];
+    aes_fill_lds(lds);
+    __syncthreads();
+    const AesCtx c = aes_ctx(lds);
+    dcf_eval_queue_body<T, CH, PRE>(c, out, x, key, seed, party, n, outBytes, N, counter);
+}
+
+// The same for workgroups below 1024 lanes, with aes_fill_lds_staged.
+template <typename T, int CH, bool PRE>
+__global__ __launch_bounds__(kBlock) void dcf_eval_queue_small_kernel(T *out, const T *x, const uint8_t *key,
+                                                                      const uint8_t *seed, int party, int n,
+                                                                      int outBytes, size_t N, uint32_t *counter)
+{
+    __shared__ __align__(16) uint32_t lds[kLdsTableWords];
+    aes_fill_lds_staged(lds);
+    __syncthreads();
+    const AesCtx c = aes_ctx(lds);
+    dcf_eval_queue_body<T, CH, PRE>(c, out, x, key, seed, party, n, outBytes, N, counter);
 }
 
 __global__ void queue_reset_kernel(uint32_t *counter)
@@ -250,8 +271,15 @@ int launch_dcf_eval_g1(int v, int grid, int block, hipStream_t st, T *out, const
     {
         uint32_t *counter = queue_counter_acquire(st);
         if (!counter) return FAST_FSS_RUNTIME_ERROR;
-        if (v == 6 && sizeof(T) <= 4)
+        const bool small = block < kBlock;
+        if (v == 6 && sizeof(T) <= 4 && small)
+            dcf_eval_queue_small_kernel<T, 8, (sizeof(T) <= 4)>
+                <<<grid, block, 0, st>>>(out, x, key, seed, party, n, outBytes, N, counter);
+        else if (v == 6 && sizeof(T) <= 4)
             dcf_eval_queue_kernel<T, 8, (sizeof(T) <= 4)>
+                <<<grid, block, 0, st>>>(out, x, key, seed, party, n, outBytes, N, counter);
+        else if (small)
+            dcf_eval_queue_small_kernel<T, 8, false>
                 <<<grid, block, 0, st>>>(out, x, key, seed, party, n, outBytes, N, counter);
         else
             dcf_eval_queue_kernel<T, 8, false>

@@ -81,11 +81,10 @@ __global__ __launch_bounds__(kBlock) void dpf_eval_burst_kernel(T *out, const T 
 
 // Work-queue form (see dcf_eval_queue_kernel): 64-element tiles per wave.
 template <typename T, int CH>
-__global__ __launch_bounds__(kBlock) void dpf_eval_queue_kernel(T *out, const T *x, const uint8_t *key,
-                                                                const uint8_t *seed, int party, int n, int m, size_t N,
-                                                                uint32_t *counter)
+__device__ __forceinline__ void dpf_eval_queue_body(const AesCtx &c, T *out, const T *x, const uint8_t *key,
+                                                    const uint8_t *seed, int party, int n, int m, size_t N,
+                                                    uint32_t *counter)
 {
-    FSS_KERNEL_PROLOGUE
     const T       *tails = (const T *)(key + N * 16 * (size_t)n);
     const uint32_t lane  = threadIdx.x & 63u;
     const uint32_t tiles = (uint32_t)((N + 63) / 64);
@@ -103,6 +102,25 @@ __global__ __launch_bounds__(kBlock) void dpf_eval_queue_kernel(T *out, const T 
             out[e] = dpf_eval_g1_burst<T, CH>(c, key + e * 16 * (size_t)n, tails + e * 3, x[e], s, party, n, m);
         }
     }
+}
+
+template <typename T, int CH>
+__global__ __launch_bounds__(kBlock) void dpf_eval_queue_kernel(T *out, const T *x, const uint8_t *key,
+                                                                const uint8_t *seed, int party, int n, int m, size_t N,
+                                                                uint32_t *counter)
+{
+    FSS_KERNEL_PROLOGUE
+    dpf_eval_queue_body<T, CH>(c, out, x, key, seed, party, n, m, N, counter);
+}
+
+// The same for workgroups below 1024 lanes, with aes_fill_lds_staged.
+template <typename T, int CH>
+__global__ __launch_bounds__(kBlock) void dpf_eval_queue_small_kernel(T *out, const T *x, const uint8_t *key,
+                                                                      const uint8_t *seed, int party, int n, int m,
+                                                                      size_t N, uint32_t *counter)
+{
+    FSS_KERNEL_PROLOGUE_STAGED
+    dpf_eval_queue_body<T, CH>(c, out, x, key, seed, party, n, m, N, counter);
 }
 
 // One work item per (element, point): out[e][j] = Eval(x_e - point_j)
@@ -708,9 +726,19 @@ extern "C" FAST_FSS_API int FastFss_cuda_dpfEval(void *sharedOut, size_t sharedO
     {
         uint32_t *counter = queue_counter_acquire(st);
         if (!counter) return FAST_FSS_RUNTIME_ERROR;
-        FSS_DISPATCH(es, dpf_eval_queue_kernel<T, 8><<<FSS_SHAPE(N), 0, st>>>(
-                             (T *)sharedOut, (const T *)maskedX, (const uint8_t *)key, (const uint8_t *)seed,
-                             partyId, (int)n, (int)m, N, counter));
+        const LaunchShape ls = shape_for(N);
+        if (ls.block < kBlock)
+        {
+            FSS_DISPATCH(es, (dpf_eval_queue_small_kernel<T, 8><<<ls.grid, ls.block, 0, st>>>(
+                                 (T *)sharedOut, (const T *)maskedX, (const uint8_t *)key, (const uint8_t *)seed,
+                                 partyId, (int)n, (int)m, N, counter)));
+        }
+        else
+        {
+            FSS_DISPATCH(es, (dpf_eval_queue_kernel<T, 8><<<ls.grid, ls.block, 0, st>>>(
+                                 (T *)sharedOut, (const T *)maskedX, (const uint8_t *)key, (const uint8_t *)seed,
+                                 partyId, (int)n, (int)m, N, counter)));
+        }
         queue_counter_release(st, counter);
     }
     else if (g == 1)
