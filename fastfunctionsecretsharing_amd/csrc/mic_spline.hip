@@ -21,6 +21,13 @@ namespace fss {
     __syncthreads();                        \
     const AesCtx c = aes_ctx(lds);
 
+// the same for kernels launched with workgroups below 1024 lanes (aes_fill_lds_staged)
+#define FSS_KERNEL_PROLOGUE_STAGED                         \
+    __shared__ __align__(16) uint32_t lds[kLdsTableWords]; \
+    aes_fill_lds_staged(lds);                              \
+    __syncthreads();                                       \
+    const AesCtx c = aes_ctx(lds);
+
 #define FSS_GRID_LOOP(i, count)                                                             \
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < (count); \
          i += (size_t)gridDim.x * blockDim.x)
@@ -97,12 +104,10 @@ __device__ __forceinline__ void mic_eval_range(const AesCtx &c, T *out, const T 
 // the GPU; the first interval of a block is evaluated without the reuse of
 // its predecessor, which gives the same value (src/kernel/mic.h:189-272).
 template <typename T>
-__global__ __launch_bounds__(kBlock) void mic_eval_kernel(T *out, const T *x, const uint8_t *key, const T *z,
-                                                          const uint8_t *seed, int party, const T *left,
-                                                          const T *right, size_t I, size_t blocksPerElem, int n,
-                                                          int outBytes, size_t N, uint32_t *counter)
+__device__ __forceinline__ void mic_eval_body(const AesCtx &c, T *out, const T *x, const uint8_t *key, const T *z,
+                                              const uint8_t *seed, int party, const T *left, const T *right, size_t I,
+                                              size_t blocksPerElem, int n, int outBytes, size_t N, uint32_t *counter)
 {
-    FSS_KERNEL_PROLOGUE
     const T     *tails = (const T *)(key + N * 16 * (size_t)n);
     const size_t span  = (I + blocksPerElem - 1) / blocksPerElem;
     for_each_item(N * blocksPerElem, counter, [&](size_t w) {
@@ -115,6 +120,27 @@ __global__ __launch_bounds__(kBlock) void mic_eval_kernel(T *out, const T *x, co
         mic_eval_range<T>(c, out + e * I, z + e * I, key + e * 16 * (size_t)n, tails + e * (size_t)(n + 3), x[e], s,
                           party, left, right, first, cnt, n, outBytes);
     });
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void mic_eval_kernel(T *out, const T *x, const uint8_t *key, const T *z,
+                                                          const uint8_t *seed, int party, const T *left,
+                                                          const T *right, size_t I, size_t blocksPerElem, int n,
+                                                          int outBytes, size_t N, uint32_t *counter)
+{
+    FSS_KERNEL_PROLOGUE
+    mic_eval_body<T>(c, out, x, key, z, seed, party, left, right, I, blocksPerElem, n, outBytes, N, counter);
+}
+
+// The same for workgroups below 1024 lanes, with aes_fill_lds_staged.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void mic_eval_small_kernel(T *out, const T *x, const uint8_t *key, const T *z,
+                                                                const uint8_t *seed, int party, const T *left,
+                                                                const T *right, size_t I, size_t blocksPerElem, int n,
+                                                                int outBytes, size_t N, uint32_t *counter)
+{
+    FSS_KERNEL_PROLOGUE_STAGED
+    mic_eval_body<T>(c, out, x, key, z, seed, party, left, right, I, blocksPerElem, n, outBytes, N, counter);
 }
 
 // --------------------------------------------------------------- spline ----
@@ -316,10 +342,18 @@ int launch_mic_eval_rt(size_t es, int grid, hipStream_t st, void *out, const voi
                        size_t N, uint32_t *counter, int block)
 {
     const size_t bpe = mic_blocks_per_elem(N, I);
-    FSS_DISPATCH(es, mic_eval_kernel<T><<<grid, block, 0, st>>>((T *)out, (const T *)x, (const uint8_t *)key,
-                                                                 (const T *)z, (const uint8_t *)seed, party,
-                                                                 (const T *)left, (const T *)right, I, bpe, n,
-                                                                 (m + 7) / 8, N, counter));
+    if (block < kBlock)
+    {
+        FSS_DISPATCH(es, mic_eval_small_kernel<T><<<grid, block, 0, st>>>(
+                             (T *)out, (const T *)x, (const uint8_t *)key, (const T *)z, (const uint8_t *)seed, party,
+                             (const T *)left, (const T *)right, I, bpe, n, (m + 7) / 8, N, counter));
+    }
+    else
+    {
+        FSS_DISPATCH(es, mic_eval_kernel<T><<<grid, block, 0, st>>>(
+                             (T *)out, (const T *)x, (const uint8_t *)key, (const T *)z, (const uint8_t *)seed, party,
+                             (const T *)left, (const T *)right, I, bpe, n, (m + 7) / 8, N, counter));
+    }
     return launch_status();
 }
 
