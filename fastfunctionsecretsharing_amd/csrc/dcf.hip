@@ -1,10 +1,11 @@
 // FastFss_cuda_dcfKeyGen / FastFss_cuda_dcfEval on MI355X.
 //
-// Launch shape: persistent grid of one 1024-thread workgroup per CU (the
-// 128 KB AES tables fill most of the CU's LDS), one element per lane per
-// iteration of a grid-stride loop.  Argument validation follows the
-// reference Task::check order (src/kernel/dcf.h:31-69, 115-163) so callers see
-// the same error codes.
+// Launch shape (shape_for): one 1024-thread workgroup per CU once the call
+// fills every CU (the 128 KB AES tables take most of the CU's LDS), smaller
+// workgroups spread over all CUs below that; one element per lane per
+// iteration of a grid-stride loop or work-queue tile.  Argument validation
+// follows the reference Task::check order (src/kernel/dcf.h:31-69, 115-163) so
+// callers see the same error codes.
 #include <atomic>
 
 #include "FastFss/cuda/dcf.h"

@@ -10,8 +10,9 @@
 
 namespace fss {
 
-// Launches use one 1024-thread workgroup per CU: the AES tables take 128 KB
-// of the CU's 160 KB LDS, so a second workgroup would not fit anyway.
+// At most one workgroup fits on a CU: the AES tables take 128 KB of its
+// 160 KB LDS.  Launches that fill the GPU use 1024-thread workgroups, one
+// per CU (shape_for gives smaller calls smaller ones).
 constexpr int kBlock = 1024;
 // groupSize > 1 keyGen workgroup size: 768 threads (three waves per SIMD
 // with the one table-holding workgroup per CU, 168 registers per lane; the
