@@ -76,6 +76,31 @@ __global__ __launch_bounds__(kBlock) void dcf_eval_g1_kernel(T *out, const T *x,
     }
 }
 
+// Two lanes per element (dcf_eval_g1_pair); the grid-stride step is even, so
+// both lanes of a pair always hold the same element.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void dcf_eval_pair_kernel(T *out, const T *x, const uint8_t *key,
+                                                               const uint8_t *seed, int party, int n, int outBytes,
+                                                               size_t N)
+{
+    __shared__ __align__(16) uint32_t lds[kLdsTableWords];
+    aes_fill_lds_staged(lds);
+    __syncthreads();
+    const AesCtx   c     = aes_ctx(lds);
+    const size_t   sw    = 16 * (size_t)n;
+    const T       *tails = (const T *)(key + N * sw);
+    const size_t   ts    = (size_t)n + 3;
+    const uint32_t odd   = threadIdx.x & 1u;
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; (g >> 1) < N; g += (size_t)gridDim.x * blockDim.x)
+    {
+        const size_t e = g >> 1;
+        uint32_t     s[4];
+        load16(s, seed + 16 * e);
+        const T v = dcf_eval_g1_pair<T>(c, key + e * sw, tails + e * ts, x[e], s, party, n, outBytes, odd);
+        if (odd) out[e] = v;
+    }
+}
+
 // E elements per lane: lane `gid` takes elements gid + k*S (k < E, S = total
 // lanes) per step of a grid-stride loop over E*S; surplus slots redo the
 // first element and are not stored.
@@ -237,14 +262,16 @@ void queue_counter_release(hipStream_t st, uint32_t *p)
 // CH levels with E elements per lane (see dcf_eval_g1_burst) over a static
 // grid-stride split, 5 = CH 8 bursts over the work queue, 6 = variant 5 with
 // each element's value corrections loaded up front (dcf_eval_g1_pre; elements
-// of at most 32 bits, 64-bit elements run variant 5).
+// of at most 32 bits, 64-bit elements run variant 5), 7 = two lanes per
+// element (dcf_eval_g1_pair; its own launch shape over 2N lanes).
 struct EvalVariant
 {
     int ch, e, queue;
 };
-constexpr EvalVariant kEvalVariants[] = {{1, 1, 0}, {4, 1, 0}, {8, 1, 0}, {4, 2, 0}, {2, 2, 0}, {8, 1, 1}, {8, 1, 2}};
+constexpr EvalVariant kEvalVariants[] = {{1, 1, 0}, {4, 1, 0}, {8, 1, 0}, {4, 2, 0},
+                                         {2, 2, 0}, {8, 1, 1}, {8, 1, 2}, {1, 1, 0}};
 constexpr int         kNumEvalVariants = sizeof(kEvalVariants) / sizeof(kEvalVariants[0]);
-constexpr size_t      kQueueMinElems   = (size_t)1 << 16; // below this the static split is used
+constexpr size_t      kQueueMinElems   = (size_t)1 << 20; // below this the static split is used
 
 // set from any host thread while others launch: atomics, relaxed (each value
 // is independent and only steers the choice of launch)
@@ -258,10 +285,15 @@ int dcf_eval_variant(size_t es, size_t n, size_t N)
     if (o >= 0) return o;
     int v = (es < 9) ? g_variant_table[es].load(std::memory_order_relaxed) : -1;
     if (v >= 0) return v;
-    // measured on MI355X: the queue form is 9-16% faster on large launches,
-    // and preloading the value corrections reads each key byte once (n = 32,
-    // 2^24: 11.3 instead of 19.0 GB fetched, 1-2% faster)
-    return N >= kQueueMinElems ? 6 : 2;
+    // measured on MI355X (n = 16/32/64): the queue form with preloaded value
+    // corrections overtakes the static split at 2^20-2^21 elements and is
+    // 9-16% faster at 2^22-2^24, where it also reads each key byte once
+    // (n = 32, 2^24: 11.3 instead of 19.0 GB fetched); below that its counter
+    // setup and per-tile atomics cost more than they balance.  Lane pairs win
+    // while they leave at most two waves per SIMD (2^16 elements: 72 against
+    // 88 us for the static split at n = 32).
+    if (N >= kQueueMinElems) return 6;
+    return 2 * N <= (size_t)device_cu_count() * 512 ? 7 : 2;
 }
 
 template <typename T>
@@ -288,6 +320,12 @@ int launch_dcf_eval_g1(int v, int grid, int block, hipStream_t st, T *out, const
         queue_counter_release(st, counter);
         return FAST_FSS_SUCCESS;
     }
+    if (v == 7)
+    {
+        const LaunchShape ps = shape_for(2 * N);
+        dcf_eval_pair_kernel<T><<<ps.grid, ps.block, 0, st>>>(out, x, key, seed, party, n, outBytes, N);
+        return FAST_FSS_SUCCESS;
+    }
     switch (v)
     {
         case 0: dcf_eval_g1_kernel<T><<<grid, block, 0, st>>>(out, x, key, seed, party, n, outBytes, N); break;
@@ -300,12 +338,12 @@ int launch_dcf_eval_g1(int v, int grid, int block, hipStream_t st, T *out, const
 }
 
 // Serving launches choose their own grid shape, so they keep the static split
-// (variants 5 and 6 fall back to variant 2 there).
+// (variants 5, 6 and 7 fall back to variant 2 there).
 int launch_dcf_eval_g1_rt(size_t es, int grid, hipStream_t st, void *out, const void *x, const void *key,
                           const void *seed, int party, int n, int m, size_t N, int block)
 {
     int v = dcf_eval_variant(es, (size_t)n, N);
-    if (v == 5 || v == 6) v = 2;
+    if (v >= 5) v = 2;
     int rc = FAST_FSS_SUCCESS;
     FSS_DISPATCH(es, rc = launch_dcf_eval_g1<T>(v, grid, block, st, (T *)out, (const T *)x, (const uint8_t *)key,
                                                 (const uint8_t *)seed, party, n, (m + 7) / 8, N));

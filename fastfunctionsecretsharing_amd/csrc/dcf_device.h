@@ -65,6 +65,50 @@ __device__ __forceinline__ T dcf_eval_g1(const AesCtx &c, const uint8_t *sCW, co
     return (T)(acc + neg_if<T>(term, party));
 }
 
+// dcf_eval_g1 on a lane pair, for calls too small to fill the GPU, where one
+// wave's instruction stream sets the time.  Both lanes run the key schedule;
+// the even lane encrypts the taken child's seed block (4x, 4x+1) and the odd
+// lane its value block (4x+2, 4x+3), so each lane issues one block per level
+// instead of two.  The odd lane takes the seed block from its partner with a
+// DPP quad permutation (a VALU operand modifier, no LDS), so both lanes carry
+// the same seed and t bit down the tree; only the odd lane's sum is the share.
+template <typename T>
+__device__ __forceinline__ T dcf_eval_g1_pair(const AesCtx &c, const uint8_t *sCW, const T *tail, T x, uint32_t s[4],
+                                              int party, int n, int outBytes, uint32_t odd)
+{
+    s[0] &= ~1u;
+    uint32_t t   = (uint32_t)party;
+    x            = mod_bits<T>(x, n);
+    T        acc = 0;
+    const T  tl = tail[n], tr = tail[n + 1];
+    for (int i = 0; i < n; i++)
+    {
+        const uint32_t xi = (uint32_t)(x >> (n - 1 - i)) & 1u;
+        uint32_t       cw[4];
+        load16(cw, sCW + 16 * i);
+        const T        vcw   = tail[i];
+        const uint32_t p0[1] = {4u * xi + 2u * odd}, p2[1] = {4u * xi + 1u + 2u * odd};
+        uint32_t       blk[1][4];
+        aes_encrypt_lowpt<1>(c, s, p0, p2, blk);
+        // quad_perm [0, 0, 2, 2]: every lane reads its pair's even lane
+        uint32_t sb[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) sb[k] = (uint32_t)__builtin_amdgcn_mov_dpp((int)blk[0][k], 0xA0, 0xF, 0xF, false);
+        const uint32_t m  = 0u - t;
+        uint32_t       tn = sb[0] & 1u;
+        s[0]              = (sb[0] & ~1u) ^ (cw[0] & m);
+        s[1]              = sb[1] ^ (cw[1] & m);
+        s[2]              = sb[2] ^ (cw[2] & m);
+        s[3]              = sb[3] ^ (cw[3] & m);
+        tn ^= (uint32_t)(((xi ? tr : tl) >> i) & 1) & t;
+        T term = (T)((T)low_bytes(blk[0], outBytes) + (t ? vcw : (T)0));
+        acc    = (T)(acc + neg_if<T>(term, party));
+        t      = tn;
+    }
+    T term = (T)((T)low_bytes(s, outBytes) + (t ? tail[n + 2] : (T)0));
+    return (T)(acc + neg_if<T>(term, party));
+}
+
 // Groupsize-1 shares of E elements per lane with the correction words read in
 // bursts of CH levels.  A lane's sCW run is 16n contiguous bytes and its vCW
 // run n contiguous group elements; fetching CH levels back to back consumes a

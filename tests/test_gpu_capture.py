@@ -1,6 +1,7 @@
 """Per-call entry points inside a captured graph.
 
-From 2^16 elements the DCF and DPF evals run on the work queue. Each call
+The DCF eval runs on the work queue from 2^20 elements (or when variant 6 is
+selected, as here) and the DPF eval from 2^21. Each call
 then takes a 4-byte tile counter from the stream-ordered allocator, clears it
 with a one-thread kernel (not a memset) and frees it on the same stream. Such
 a call can be recorded into a graph. Each test captures one call, replays the
@@ -55,7 +56,8 @@ def test_queue_eval_in_graph(env, op):
     gen = torch.Generator(device=dev).manual_seed(61)
     holder, sp0 = W.stream_ptr()
     kind = _lib.BATCH_DCF_EVAL if op == "dcf" else _lib.BATCH_DPF_EVAL
-    g = W.make_group(kind, 32, (1 << 16) + 77, gen, dev, sp0)  # queue form, partial last tile
+    # queue form (DPF from 2^21 elements, DCF with variant 6 below), partial last tile
+    g = W.make_group(kind, 32, (1 << (16 if op == "dcf" else 21)) + 77, gen, dev, sp0)
     torch.cuda.synchronize()
     out = torch.empty(g.N, dtype=g.alpha.dtype, device=dev)
     fn = lib.FastFss_cuda_dcfEval if op == "dcf" else lib.FastFss_cuda_dpfEval
@@ -65,5 +67,10 @@ def test_queue_eval_in_graph(env, op):
                 g.seeds[1].data_ptr(), 16 * g.N, 1, 32, 32, 1, 4, g.N, None, 0, sp)
         assert rc == 0, rc
 
-    _replays_match(torch, call, out)
+    if op == "dcf":  # the DCF default moves to the queue only from 2^20 elements
+        assert lib.FastFss_hip_setEvalVariant(6) == 0
+    try:
+        _replays_match(torch, call, out)
+    finally:
+        lib.FastFss_hip_setEvalVariant(-1)
     del holder

@@ -97,8 +97,8 @@ def test_dpf_eval_all_subtrees(dev, n, m, es, g, lanes):
 
 @pytest.mark.parametrize("es,n,m", [(2, 16, 8), (4, 32, 32), (8, 64, 20)])
 def test_dpf_queue_size(dev, es, n, m):
-    """From 2^16 elements the groupSize-1 DPF eval runs on the work queue; 70001
-    elements end in a partial tile."""
+    """70001 elements: the static split with a partial last stride (the work
+    queue starts at 2^21 elements, test_dpf_queue_2pow21)."""
     rng = np.random.default_rng(es + n)
     N = 70001
     alpha = rand_elems(rng, N, n, es)
@@ -112,6 +112,38 @@ def test_dpf_queue_size(dev, es, n, m):
     for party, sd in ((0, s0), (1, s1)):
         assert np.array_equal(dev.get(dpf_eval(dev, x, kd, sd, party, n, m, 1, es)),
                               dpf_eval(OR, x, ko, sd, party, n, m, 1, es)), party
+
+
+def test_dpf_queue_2pow21(dev):
+    """From 2^21 elements the groupSize-1 DPF eval runs on the work queue: 2^21 + 77
+    elements (n = m = 32) end in a partial tile.  Every element reconstructs to
+    beta * [x == alpha]; a sample of 2048 keys and shares equals the oracle's."""
+    import torch
+
+    N, n, m, es = (1 << 21) + 77, 32, 32, 4
+    gen = torch.Generator(device="cuda:0").manual_seed(21)
+    alpha = torch.randint(0, 2**32, (N,), dtype=torch.int64, device="cuda:0", generator=gen)
+    x = torch.randint(0, 2**32, (N,), dtype=torch.int64, device="cuda:0", generator=gen)
+    x[::5] = alpha[::5]
+    beta = torch.randint(0, 2**32, (N,), dtype=torch.int64, device="cuda:0", generator=gen)
+    seeds = torch.randint(0, 256, (2, 16 * N), dtype=torch.uint8, device="cuda:0", generator=gen)
+    a32, x32, b32 = alpha.to(torch.int32), x.to(torch.int32), beta.to(torch.int32)
+    key = dpf_keygen(dev, a32, b32, seeds[0], seeds[1], n, m, 1, es)
+    y0 = dpf_eval(dev, x32, key, seeds[0], 0, n, m, 1, es)
+    y1 = dpf_eval(dev, x32, key, seeds[1], 1, n, m, 1, es)
+    rec = (y0.to(torch.int64) + y1.to(torch.int64)) & 0xFFFFFFFF
+    assert torch.equal(rec, torch.where(x == alpha, beta, torch.zeros_like(beta)))
+    idx = np.sort(np.random.default_rng(7).choice(N, 2048, replace=False))
+    kh = dev.get(key)
+    sub = np.concatenate([kh[:N * 16 * n].reshape(N, 16 * n)[idx].reshape(-1),
+                          kh[N * 16 * n:].reshape(N, es * 3)[idx].reshape(-1)])
+    s0 = dev.get(seeds[0]).reshape(N, 16)[idx].reshape(-1)
+    s1 = dev.get(seeds[1]).reshape(N, 16)[idx].reshape(-1)
+    ko = dpf_keygen(OR, dev.get(a32)[idx], dev.get(b32)[idx], s0, s1, n, m, 1, es)
+    assert np.array_equal(ko, sub)
+    xh = dev.get(x32)[idx]
+    assert np.array_equal(dpf_eval(OR, xh, ko, s0, 0, n, m, 1, es), dev.get(y0)[idx])
+    assert np.array_equal(dpf_eval(OR, xh, ko, s1, 1, n, m, 1, es), dev.get(y1)[idx])
 
 
 # --------------------------------------------------------------- Grotto ----
