@@ -196,7 +196,12 @@ __global__ __launch_bounds__(group_keygen_block(sizeof(T), OB)) void spline_keyg
 // intervals q and q+1 are adjacent (the reference copies the whole vector,
 // src/impl/spline.h:214-223).  OB > 0 selects the slot conversion specialised
 // for ceil(bitWidthOut/8) = OB bytes (needs 2*(degree+1)*OB <= 32); OB = 0 the
-// generic one.
+// generic one.  An element's intervals may be split into bpe blocks (a power
+// of two dividing 64) on consecutive lanes: each lane sums its intervals'
+// terms of the polynomial, the first interval of a block walks its own start
+// point instead of taking it from its predecessor's walk (the same value), and
+// the bpe partial sums are added across the lanes, whose results are equal to
+// the one-lane walk's mod 2^m.
 template <typename T, int OB>
 __device__ __forceinline__ void spline_walk(const AesCtx &c, T *acc, const uint8_t *sc, const T *tail, T pt,
                                             const uint32_t s[4], int party, int n, int m, int g, int j0, int cnt)
@@ -207,18 +212,22 @@ __device__ __forceinline__ void spline_walk(const AesCtx &c, T *acc, const uint8
         dcf_eval_slots<T, OB, (32 / OB < 16 ? 32 / OB : 16)>(c, acc, sc, tail, pt, s, party, n, g, j0, cnt);
 }
 
-template <typename T, int OB>
-__global__ __launch_bounds__(kBlock) void spline_eval_kernel(T *out, const T *x, const uint8_t *key, const T *sE,
-                                                             const T *sBeta, const uint8_t *seed, int party,
-                                                             const T *left, const T *right, size_t I, int d, int n,
-                                                             int m, size_t N, uint32_t *counter)
+template <typename T, int OB, bool SPLIT>
+__device__ __forceinline__ void spline_eval_body(const AesCtx &c, T *out, const T *x, const uint8_t *key, const T *sE,
+                                                 const T *sBeta, const uint8_t *seed, int party, const T *left,
+                                                 const T *right, size_t I, int d, int n, int m, size_t N, uint32_t bpe,
+                                                 uint32_t *counter)
 {
     constexpr int RS = (OB == 0) ? 16 : (32 / OB < 16 ? 32 / OB : 16); // slots per walk
-    FSS_KERNEL_PROLOGUE
+    if constexpr (!SPLIT) bpe = 1;
     const int    C     = d + 1;
     const size_t g     = I * (size_t)C;
     const T     *tails = (const T *)(key + N * 16 * (size_t)n);
-    for_each_item(N, counter, [&](size_t e) {
+    const size_t span  = (I + bpe - 1) / bpe;
+    for_each_item(N * bpe, counter, [&](size_t item) {
+        const size_t e  = item / bpe;
+        const size_t q0 = (item % bpe) * span;
+        const size_t q1 = (q0 + span < I) ? q0 + span : I;
         uint32_t s[4];
         load16(s, seed + 16 * e);
         const uint8_t *sc   = key + e * 16 * (size_t)n;
@@ -230,19 +239,19 @@ __global__ __launch_bounds__(kBlock) void spline_eval_kernel(T *out, const T *x,
         T w[8];
 #pragma unroll
         for (int j = 0; j < 8; j++) w[j] = 0;
-        for (size_t q = 0; q < I; q++)
+        for (size_t q = q0; q < q1; q++)
         {
             const T L  = mod_bits<T>(left[q], n), R = mod_bits<T>(right[q], n);
             const T Rp = mod_bits<T>((T)(R + 1), n);
             T       acc[RS];
-            if (!(q > 0 && mod_bits<T>((T)(right[q - 1] + 1), n) == L))
+            if (!(q > q0 && mod_bits<T>((T)(right[q - 1] + 1), n) == L))
             {
                 spline_walk<T, OB>(c, acc, sc, tail, (T)(xv - 1 - L), s, party, n, m, (int)g, (int)(q * C), C);
 #pragma unroll
                 for (int j = 0; j < 8; j++)
                     if (j < C) w[j] = (T)(w[j] - acc[j < RS ? j : 0]);
             }
-            const bool link = (q + 1 < I) && mod_bits<T>((T)(right[q] + 1), n) == mod_bits<T>(left[q + 1], n);
+            const bool link = (q + 1 < q1) && mod_bits<T>((T)(right[q] + 1), n) == mod_bits<T>(left[q + 1], n);
             spline_walk<T, OB>(c, acc, sc, tail, (T)(xv - 1 - Rp), s, party, n, m, (int)g, (int)(q * C),
                                link ? 2 * C : C);
             const T cx = (T)((int)(xv > L) - (int)(xv > Rp));
@@ -276,8 +285,39 @@ __global__ __launch_bounds__(kBlock) void spline_eval_kernel(T *out, const T *x,
                 pw = (T)(pw * xv);
             }
         }
-        out[e] = mod_bits<T>(r, m);
+        // the element's bpe lanes are consecutive within one wave and all active
+        for (uint32_t o = 1; o < bpe; o <<= 1)
+        {
+            const uint64_t rv = (uint64_t)r;
+            const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)rv, (int)o);
+            const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(rv >> 32), (int)o);
+            r                 = (T)(r + (T)((uint64_t)lo | ((uint64_t)hi << 32)));
+        }
+        if (item % bpe == 0) out[e] = mod_bits<T>(r, m);
     });
+}
+
+template <typename T, int OB>
+__global__ __launch_bounds__(kBlock) void spline_eval_kernel(T *out, const T *x, const uint8_t *key, const T *sE,
+                                                             const T *sBeta, const uint8_t *seed, int party,
+                                                             const T *left, const T *right, size_t I, int d, int n,
+                                                             int m, size_t N, uint32_t *counter)
+{
+    FSS_KERNEL_PROLOGUE
+    spline_eval_body<T, OB, false>(c, out, x, key, sE, sBeta, seed, party, left, right, I, d, n, m, N, 1, counter);
+}
+
+// The same with each element's intervals split into bpe > 1 blocks (calls
+// too small to give every CU a full workgroup).
+template <typename T, int OB>
+__global__ __launch_bounds__(kBlock) void spline_eval_split_kernel(T *out, const T *x, const uint8_t *key,
+                                                                   const T *sE, const T *sBeta, const uint8_t *seed,
+                                                                   int party, const T *left, const T *right, size_t I,
+                                                                   int d, int n, int m, size_t N, uint32_t bpe,
+                                                                   uint32_t *counter)
+{
+    FSS_KERNEL_PROLOGUE_STAGED
+    spline_eval_body<T, OB, true>(c, out, x, key, sE, sBeta, seed, party, left, right, I, d, n, m, N, bpe, counter);
 }
 
 // Any degree: the reference's per-element workspace inside the caller's
@@ -337,6 +377,18 @@ static size_t mic_blocks_per_elem(size_t N, size_t I)
     return bpe;
 }
 
+// Interval blocks per element for the spline walk: a power of two dividing
+// 64 (so an element's lanes share one wave), at most I, and enough lanes to
+// give every CU a full workgroup.
+static uint32_t spline_blocks_per_elem(size_t N, size_t I)
+{
+    const size_t lanes = (size_t)device_cu_count() * kBlock;
+    const size_t want  = (lanes + N - 1) / N;
+    uint32_t     b     = 1;
+    while (b * 2 <= 64 && b * 2 <= want && b * 2 <= I) b *= 2;
+    return b;
+}
+
 int launch_mic_eval_rt(size_t es, int grid, hipStream_t st, void *out, const void *x, const void *key, const void *z,
                        const void *seed, int party, const void *left, const void *right, size_t I, int n, int m,
                        size_t N, uint32_t *counter, int block)
@@ -362,9 +414,16 @@ static void spline_launch_ob(int grid, int block, hipStream_t st, void *out, con
                              const void *beta, const void *seed, int party, const void *left, const void *right,
                              size_t I, int degree, int n, int m, size_t N, uint32_t *counter)
 {
-    spline_eval_kernel<T, OB><<<grid, block, 0, st>>>((T *)out, (const T *)x, (const uint8_t *)key, (const T *)e,
-                                                       (const T *)beta, (const uint8_t *)seed, party, (const T *)left,
-                                                       (const T *)right, I, degree, n, m, N, counter);
+    const uint32_t bpe = spline_blocks_per_elem(N, I);
+    if (bpe > 1)
+        spline_eval_split_kernel<T, OB><<<grid, block, 0, st>>>(
+            (T *)out, (const T *)x, (const uint8_t *)key, (const T *)e, (const T *)beta, (const uint8_t *)seed, party,
+            (const T *)left, (const T *)right, I, degree, n, m, N, bpe, counter);
+    else
+        spline_eval_kernel<T, OB><<<grid, block, 0, st>>>((T *)out, (const T *)x, (const uint8_t *)key, (const T *)e,
+                                                           (const T *)beta, (const uint8_t *)seed, party,
+                                                           (const T *)left, (const T *)right, I, degree, n, m, N,
+                                                           counter);
 }
 
 // degree <= 7: the slot-conversion kernel for OB = ceil(m/8) when a linked
@@ -556,7 +615,7 @@ extern "C" FAST_FSS_API int FastFss_cuda_dcfSplineEval(
     {
         uint32_t *counter = queue_counter_acquire(st);
         if (!counter) return FAST_FSS_RUNTIME_ERROR;
-        const LaunchShape ls = shape_for(N);
+        const LaunchShape ls = shape_for(N * spline_blocks_per_elem(N, I));
         const int rc = launch_spline_eval_rt(es, ls.grid, st, sharedOut, maskedX, key, sharedE, sharedBeta, seed,
                                              partyId, leftEndpoints, rightEndpoints, I, (int)degree, (int)n, (int)m,
                                              N, counter, ls.block);
