@@ -414,7 +414,10 @@ static void spline_launch_ob(int grid, int block, hipStream_t st, void *out, con
                              const void *beta, const void *seed, int party, const void *left, const void *right,
                              size_t I, int degree, int n, int m, size_t N, uint32_t *counter)
 {
-    const uint32_t bpe = spline_blocks_per_elem(N, I);
+    // split only as far as the launch has lanes for (the serving runtime sizes
+    // its grid by elements; a split it has no lanes for only adds walks)
+    uint32_t bpe = spline_blocks_per_elem(N, I);
+    while (bpe > 1 && N * bpe > (size_t)grid * (size_t)block) bpe >>= 1;
     if (bpe > 1)
         spline_eval_split_kernel<T, OB><<<grid, block, 0, st>>>(
             (T *)out, (const T *)x, (const uint8_t *)key, (const T *)e, (const T *)beta, (const uint8_t *)seed, party,
