@@ -30,6 +30,10 @@ int launch_mic_eval_rt(size_t es, int grid, hipStream_t st, void *out, const voi
                        const void *seed, int party, const void *left, const void *right, size_t I, int n, int m,
                        size_t N, uint32_t *counter, int block = kBlock);
 
+// Interval blocks per element of the spline walk (1 when the call fills the
+// GPU); launchers that size their grid from the work multiply N by it.
+uint32_t spline_blocks_per_elem(size_t N, size_t I);
+
 // degree <= 7 (the path that needs no per-element workspace)
 int launch_spline_eval_rt(size_t es, int grid, hipStream_t st, void *out, const void *x, const void *key,
                           const void *e, const void *beta, const void *seed, int party, const void *left,

@@ -380,7 +380,7 @@ static size_t mic_blocks_per_elem(size_t N, size_t I)
 // Interval blocks per element for the spline walk: a power of two dividing
 // 64 (so an element's lanes share one wave), at most I, and enough lanes to
 // give every CU a full workgroup.
-static uint32_t spline_blocks_per_elem(size_t N, size_t I)
+uint32_t spline_blocks_per_elem(size_t N, size_t I)
 {
     const size_t lanes = (size_t)device_cu_count() * kBlock;
     const size_t want  = (lanes + N - 1) / N;

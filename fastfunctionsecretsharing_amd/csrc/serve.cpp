@@ -99,6 +99,8 @@ static int launch(Server *s, ServeItem *it, hipStream_t st)
     if (r.elementNum == 0) return FAST_FSS_SUCCESS;
     size_t work = r.elementNum;
     if (r.op == FAST_FSS_SERVE_MIC_EVAL) work *= r.intervalNum > 4 ? 4 : r.intervalNum;
+    // mode 0 gives a small spline call the per-call entry point's interval split
+    if (r.op == FAST_FSS_SERVE_SPLINE_EVAL && s->launchMode == 0) work *= spline_blocks_per_elem(work, r.intervalNum);
     // mode 1: one full workgroup per 1024 elements; mode 0: the per-call shape
     LaunchShape ls = s->launchMode == 1 ? LaunchShape{(int)((work + kBlock - 1) / kBlock), kBlock} : shape_for(work);
     if (ls.grid > 65535) ls.grid = 65535;
