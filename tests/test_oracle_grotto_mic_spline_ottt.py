@@ -9,9 +9,16 @@ Checks (none of which re-derives a share with the oracle's own code):
   against their plaintext definitions (tests/grotto/grotto_mic_eval_test_case.h).
 * DCF-MIC: share0 + share1 == [x in [p, q]] for signed n-bit intervals
   (tests/mic/mic_eval_test_case.h:84-106) and z from its closed form.
+* Grotto interval LUT: E*T == LUT[matched interval] (0 where none matches) on
+  the reference's signed full partition and on partitions with gaps, with E a
+  sign (+1/-1) and T == E*LUT (tests/grotto/grotto_interval_lut_eval_cpu.cpp:
+  107-116, grotto_interval_lut_eval_test_case.h:71-119).
 * Spline: reconstruction == f_i(x) for the interval holding x
   (tests/spline/spline_eval_test_case.h:128-152); the shifted coefficients beta
-  satisfy f(y - alpha) == sum beta_k y^k at random y; e == cr * beta.
+  satisfy f(y - alpha) == sum beta_k y^k at random y; e == cr * beta. For
+  bitWidthIn != bitWidthOut the reconstruction is f_q(X - alpha) mod 2^out at
+  the integer difference X - alpha, X = maskedX mod 2^in, q the interval holding
+  (X - alpha) mod 2^in (src/impl/spline.h:96,159-161,206,252).
 * OTTT: E*T == LUT[x] (tests/ottt/ottt_eval_cpu.cpp:83-92) and the key differs
   from the caller's mask only in bit alpha.
 """
@@ -125,6 +132,66 @@ def test_grotto_mic_and_lut(n, es):
         del bits
 
 
+def signed_full_partition(n, I):
+    """The signed n-bit range cut into I contiguous intervals, as unsigned n-bit
+    patterns (tests/common/nbit_signed.hpp:136-211): interval k covers offsets
+    [2^n k / I, 2^n (k+1) / I - 1] from -2^(n-1)."""
+    dom, umin, M = 1 << n, 1 << (n - 1), mask(n)
+    left = [(umin + dom * k // I) & M for k in range(I)]
+    right = [(umin + dom * (k + 1) // I - 1) & M for k in range(I)]
+    return left, right
+
+
+@pytest.mark.parametrize("n,m,es,I,lutNum,zero_alpha,gaps", [
+    (6, 8, 1, 3, 2, True, False),     # the reference's grid (grotto_interval_lut_eval_cpu.cpp:141-163)
+    (6, 16, 2, 3, 2, True, False),
+    (6, 32, 4, 3, 2, True, False),
+    (6, 64, 8, 3, 2, True, False),
+    (8, 8, 1, 5, 3, False, False),
+    (12, 16, 2, 7, 2, False, True),
+    (20, 32, 4, 4, 2, False, True),
+    (33, 64, 8, 6, 4, False, True),
+])
+def test_grotto_interval_lut_reconstruction(n, m, es, I, lutNum, zero_alpha, gaps):
+    """E*T reconstructs LUT[matched] for both parties together: E = E0 + E1 is +1 or -1 where an
+    interval matches and 0 where none does, and T = T0 + T1 == E * LUT[matched]. A swapped LUT
+    index, a missing party sign or a dropped party-1 term fails one of these."""
+    rng = np.random.default_rng(1000 * n + m)
+    N = 600
+    left, right = signed_full_partition(n, I)
+    if gaps:  # drop every third interval: points in them match nothing
+        keep = [k for k in range(I) if k % 3 != 1]
+        left, right = [left[k] for k in keep], [right[k] for k in keep]
+        I = len(left)
+    left = np.array(left, dtype=DT[es])
+    right = np.array(right, dtype=DT[es])
+    alpha = np.zeros(N, dtype=DT[es]) if zero_alpha else rand_elems(rng, N, n, es)
+    x = rand_elems(rng, N, n, es)
+    x[:I], x[I:2 * I] = left, right  # every endpoint
+    masked = ((x.astype(np.uint64) + alpha) & np.uint64(mask(n))).astype(DT[es])
+    lut = rand_elems(rng, I * lutNum, m, es)
+    s0, s1 = rand_seeds(rng, N), rand_seeds(rng, N)
+    key = grotto_keygen(BE, alpha, s0, s1, n, es)
+    E0, T0 = grotto_lut_eval(BE, masked, key, s0, 0, left, right, lut, n, m, es)
+    E1, T1 = grotto_lut_eval(BE, masked, key, s1, 1, left, right, lut, n, m, es)
+    W, Mo = mask(8 * es), mask(m)
+    matched_some = unmatched_some = False
+    for i in range(N):
+        xs = signed_nbit(int(x[i]), n)
+        hits = [k for k in range(I) if signed_nbit(int(left[k]), n) <= xs <= signed_nbit(int(right[k]), n)]
+        assert len(hits) <= 1
+        e = (int(E0[i]) + int(E1[i])) & W
+        for j in range(lutNum):
+            t = (int(T0[i * lutNum + j]) + int(T1[i * lutNum + j])) & Mo
+            want = int(lut[hits[0] + j * I]) if hits else 0
+            assert (e * t) & Mo == want, (i, j)
+            assert t == (e * want) & Mo, (i, j)
+        assert (e in (1, W)) if hits else e == 0, i
+        matched_some |= bool(hits)
+        unmatched_some |= not hits
+    assert matched_some and (unmatched_some or not gaps)
+
+
 def test_grotto_errors():
     z = np.zeros(4096, dtype=np.uint8)
     ks = 16 * 2 + 8 + 2
@@ -230,6 +297,62 @@ def test_spline_reconstruction(n, m, es, I, d):
         c = [int(v) for v in coeff[q * (d + 1):(q + 1) * (d + 1)]]
         want = sum(ck * xv ** k for k, ck in enumerate(c)) & Mo
         assert int(rec[i]) == want, i
+
+
+def _unsigned_intervals_with_gap(rng, I, n, es):
+    """Contiguous unsigned cover of [0, 2^n) (as _spline_case), then interval 1 removed, so
+    points inside it match no interval and the next interval's left end is not reused."""
+    left, right, _ = _spline_case(rng, I + 1, 0, n, n, es)
+    keep = [k for k in range(I + 1) if k != 1]
+    return left[keep].copy(), right[keep].copy()
+
+
+@pytest.mark.parametrize("n,m,es,I,d,gap", [(24, 32, 4, 16, 2, False), (24, 32, 4, 6, 2, True),
+                                            (12, 16, 2, 5, 2, False), (8, 16, 2, 4, 3, True),
+                                            (12, 10, 2, 5, 2, False), (20, 64, 8, 3, 2, True)])
+def test_spline_in_ne_out(n, m, es, I, d, gap):
+    """bitWidthIn != bitWidthOut: the coefficients are shifted by alpha mod 2^out and evaluated at
+    X = maskedX mod 2^in, so the two parties reconstruct f_q(X - alpha) mod 2^out at the integer
+    X - alpha (negative when the mask wrapped, X < alpha), q being the interval that holds the
+    unmasked x = (X - alpha) mod 2^in, and 0 when no interval holds it. Shifting mod 2^in, or
+    evaluating at x instead of X, fails this for X < alpha when n < m."""
+    rng = np.random.default_rng(7 * n + m + I)
+    N = 256
+    if gap:
+        left, right = _unsigned_intervals_with_gap(rng, I, n, es)
+    else:
+        left, right, _ = _spline_case(rng, I, d, n, m, es)
+    I = len(left)
+    coeff = rand_elems(rng, I * (d + 1), m, es)
+    alpha = rand_elems(rng, N, n, es)
+    x = rand_elems(rng, N, n, es)
+    x[:I], x[I:2 * I] = left, right
+    M = mask(n)
+    X = ((x.astype(np.uint64) + alpha) & np.uint64(M)).astype(DT[es])
+    s0, s1 = rand_seeds(rng, N), rand_seeds(rng, N)
+    key, e, beta = spline_keygen(BE, alpha, s0, s1, coeff, d, left, right, n, m, es)
+    W, Mo = mask(8 * es), mask(m)
+    b0 = rand_elems(rng, len(beta), 8 * es, es)
+    e0 = rand_elems(rng, len(e), 8 * es, es)
+    b1 = ((beta.astype(np.uint64) - b0) & np.uint64(W)).astype(DT[es])
+    e1 = ((e.astype(np.uint64) - e0) & np.uint64(W)).astype(DT[es])
+    y0 = spline_eval(BE, X, key, e0, b0, s0, 0, left, right, d, n, m, es)
+    y1 = spline_eval(BE, X, key, e1, b1, s1, 1, left, right, d, n, m, es)
+    wrapped = 0
+    for i in range(N):
+        Xi, a = int(X[i]), int(alpha[i])
+        xi = (Xi - a) & M
+        assert xi == int(x[i])
+        hits = [k for k in range(I) if int(left[k]) <= xi <= int(right[k])]
+        assert len(hits) <= 1
+        if hits:
+            c = [int(v) for v in coeff[hits[0] * (d + 1):(hits[0] + 1) * (d + 1)]]
+            want = sum(ck * (Xi - a) ** k for k, ck in enumerate(c)) & Mo
+        else:
+            want = 0
+        assert (int(y0[i]) + int(y1[i])) & Mo == want, (i, Xi, a)
+        wrapped += Xi < a
+    assert 0 < wrapped < N
 
 
 def test_spline_cache_required():
