@@ -56,12 +56,24 @@ def _compile(src: Path) -> Path:
     return obj
 
 
-def build(jobs: int | None = None, verbose: bool = True) -> Path:
+def _up_to_date(src: Path) -> bool:
+    obj = BUILD / (src.name + ".o")
+    if not obj.exists():
+        return False
+    deps = [src, *CSRC.glob("*.h"), *(REPO / "include").rglob("*.h"), Path(__file__)]
+    return obj.stat().st_mtime >= max(d.stat().st_mtime for d in deps)
+
+
+def build(jobs: int | None = None, verbose: bool = True, incremental: bool = False) -> Path:
+    """Compile every source (or, with ``incremental``, those whose object is older
+    than the source, any header or this file) and link the library."""
     BUILD.mkdir(exist_ok=True)
     srcs = sources()
     jobs = jobs or min(len(srcs), max(1, (os.cpu_count() or 4)))
+    todo = [s for s in srcs if not (incremental and _up_to_date(s))]
     with cf.ThreadPoolExecutor(jobs) as ex:
-        objs = list(ex.map(_compile, srcs))
+        list(ex.map(_compile, todo))
+    objs = [BUILD / (s.name + ".o") for s in srcs]
     tmp = OUT.with_suffix(".so.tmp")
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -76,9 +88,11 @@ def build(jobs: int | None = None, verbose: bool = True) -> Path:
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("-j", type=int, default=None)
+    ap.add_argument("--incremental", action="store_true",
+                    help="recompile only sources whose object is older than the source or any header")
     a = ap.parse_args(argv)
     try:
-        build(a.j)
+        build(a.j, incremental=a.incremental)
     except RuntimeError as e:
         print(e, file=sys.stderr)
         return 1

@@ -131,8 +131,26 @@ def test_dcf_cpu_workload_n16_2pow16(dev):
     _check(dev, alpha, None, x, s0, s1, n, m, 1, es)
 
 
+def full_size_chunks(N, sk, tail, key_h, gen_key, evals, chunk=1 << 22, what=""):
+    """Compares a full-size device key set and its shares with the oracle, chunk by chunk.
+
+    key_h is the device key (bytes, layout [sCW: N x sk][tail: N x tail]); gen_key(sl) makes the
+    oracle's key for the elements of slice sl, and evals is a list of (oracle_eval(sl, key), device
+    shares) pairs."""
+    for c0 in range(0, N, chunk):
+        c1 = min(N, c0 + chunk)
+        sl = slice(c0, c1)
+        ko = gen_key(sl)
+        kd = np.concatenate([key_h[sk * c0:sk * c1], key_h[N * sk + tail * c0:N * sk + tail * c1]])
+        assert np.array_equal(ko, kd), f"{what} key bytes differ in elements [{c0}, {c1})"
+        for j, (ev, got) in enumerate(evals):
+            assert np.array_equal(ev(sl, ko), got[sl]), f"{what} party {j} shares differ in [{c0}, {c1})"
+        print(f"{what} elements [{c0}, {c1}): keys and both parties' shares equal the oracle's", flush=True)
+
+
 def test_dcf_2pow24_n32(dev):
-    """Plain DCF, bitWidthIn = bitWidthOut = 32, 2^24 elements, one key each."""
+    """Plain DCF, bitWidthIn = bitWidthOut = 32, 2^24 elements, one key each: every key byte and both
+    parties' shares over all 2^24 elements equal the oracle's, and share0 + share1 == [x < alpha]."""
     import torch
 
     N, n, m, g, es = 1 << 24, 32, 32, 1, 4
@@ -149,15 +167,13 @@ def test_dcf_2pow24_n32(dev):
     rec = (y0.to(torch.int64) + y1.to(torch.int64)) & 0xFFFFFFFF
     want = (masked < alpha).to(torch.int64)
     assert torch.equal(rec, want)
-    # bit-exact against the oracle on a sample of 4096 elements
-    idx = np.sort(np.random.default_rng(5).choice(N, 4096, replace=False))
-    key_h = dev.get(key)
-    sub = _sub_key(key_h, idx, n, g, es, N)
-    a_h = dev.get(a32)[idx]
-    s0_h = dev.get(seeds[0]).reshape(N, 16)[idx].reshape(-1)
-    s1_h = dev.get(seeds[1]).reshape(N, 16)[idx].reshape(-1)
-    key_o = dcf_keygen(OR, a_h, None, s0_h, s1_h, n, m, g, es)
-    assert np.array_equal(key_o, sub)
-    m_h = dev.get(m32)[idx]
-    assert np.array_equal(dcf_eval(OR, m_h, key_o, s0_h, 0, n, m, g, es), dev.get(y0)[idx])
-    assert np.array_equal(dcf_eval(OR, m_h, key_o, s1_h, 1, n, m, g, es), dev.get(y1)[idx])
+    del rec, want, x, alpha, masked
+    key_h, a_h, m_h = dev.get(key), dev.get(a32), dev.get(m32)
+    s0_h, s1_h = dev.get(seeds[0]), dev.get(seeds[1])
+    y0_h, y1_h = dev.get(y0), dev.get(y1)
+    del key
+    sd = lambda s, sl: s[16 * sl.start:16 * sl.stop]
+    full_size_chunks(N, 16 * n, (n * g + 2 + g) * es, key_h,
+                     lambda sl: dcf_keygen(OR, a_h[sl], None, sd(s0_h, sl), sd(s1_h, sl), n, m, g, es),
+                     [(lambda sl, ko: dcf_eval(OR, m_h[sl], ko, sd(s0_h, sl), 0, n, m, g, es), y0_h),
+                      (lambda sl, ko: dcf_eval(OR, m_h[sl], ko, sd(s1_h, sl), 1, n, m, g, es), y1_h)], what="DCF")

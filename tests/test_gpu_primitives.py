@@ -114,6 +114,39 @@ def test_dpf_queue_size(dev, es, n, m):
                               dpf_eval(OR, x, ko, sd, party, n, m, 1, es)), party
 
 
+def test_dpf_2pow24_n32(dev):
+    """Plain DPF, bitWidthIn = bitWidthOut = 32, 2^24 elements, one key each, random beta: every key
+    byte and both parties' shares over all 2^24 elements equal the oracle's, and share0 + share1 ==
+    beta * [x == alpha] (x == alpha on every fifth element)."""
+    import torch
+
+    from test_gpu_dcf import full_size_chunks
+
+    N, n, m, es = 1 << 24, 32, 32, 4
+    gen = torch.Generator(device="cuda:0").manual_seed(24)
+    alpha = torch.randint(0, 2**32, (N,), dtype=torch.int64, device="cuda:0", generator=gen)
+    x = torch.randint(0, 2**32, (N,), dtype=torch.int64, device="cuda:0", generator=gen)
+    x[::5] = alpha[::5]
+    beta = torch.randint(0, 2**32, (N,), dtype=torch.int64, device="cuda:0", generator=gen)
+    seeds = torch.randint(0, 256, (2, 16 * N), dtype=torch.uint8, device="cuda:0", generator=gen)
+    a32, x32, b32 = alpha.to(torch.int32), x.to(torch.int32), beta.to(torch.int32)
+    key = dpf_keygen(dev, a32, b32, seeds[0], seeds[1], n, m, 1, es)
+    y0 = dpf_eval(dev, x32, key, seeds[0], 0, n, m, 1, es)
+    y1 = dpf_eval(dev, x32, key, seeds[1], 1, n, m, 1, es)
+    rec = (y0.to(torch.int64) + y1.to(torch.int64)) & 0xFFFFFFFF
+    assert torch.equal(rec, torch.where(x == alpha, beta, torch.zeros_like(beta)))
+    del rec, alpha, x, beta
+    key_h, a_h, x_h, b_h = dev.get(key), dev.get(a32), dev.get(x32), dev.get(b32)
+    s0_h, s1_h = dev.get(seeds[0]), dev.get(seeds[1])
+    y0_h, y1_h = dev.get(y0), dev.get(y1)
+    del key
+    sd = lambda s, sl: s[16 * sl.start:16 * sl.stop]
+    full_size_chunks(N, 16 * n, 3 * es, key_h,
+                     lambda sl: dpf_keygen(OR, a_h[sl], b_h[sl], sd(s0_h, sl), sd(s1_h, sl), n, m, 1, es),
+                     [(lambda sl, ko: dpf_eval(OR, x_h[sl], ko, sd(s0_h, sl), 0, n, m, 1, es), y0_h),
+                      (lambda sl, ko: dpf_eval(OR, x_h[sl], ko, sd(s1_h, sl), 1, n, m, 1, es), y1_h)], what="DPF")
+
+
 def test_dpf_queue_2pow21(dev):
     """From 2^21 elements the groupSize-1 DPF eval runs on the work queue: 2^21 + 77
     elements (n = m = 32) end in a partial tile.  Every element reconstructs to
