@@ -68,6 +68,11 @@ def profiled_lds_per_eval(log2n: int):
     return None
 
 
+def progress(msg: str):
+    """One line on stderr per finished leg (stdout carries only the JSON line)."""
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,15 +93,30 @@ def parse():
     return ap.parse_args()
 
 
+def host_threads():
+    """Threads the CPU baseline runs on: the CPU share this process is given (OMP_NUM_THREADS when
+    set, else the CPUs in its affinity mask), and the CPUs the host shows."""
+    visible = os.cpu_count() or 1
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = visible
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    share = int(env) if env.isdigit() and int(env) > 0 else allowed
+    return min(share, allowed), visible
+
+
 def cpu_baseline(log2n: int):
-    """Time the oracle (C port of the reference CPU path) on this host."""
+    """Time the oracle (C port of the reference CPU path) on this host, on the headline workload
+    (DCF n = 32, a 2^log2n-element sample) and on the reference's CPU-sized workload (DCF n = 16 in
+    64-bit elements, 2^16 elements, beta = NULL, both parties) in full."""
     import numpy as np
 
     sys.path.insert(0, str(REPO / "tests"))
     from fss_testlib import OracleBackend, dcf_keygen, ocall, oracle, rand_elems, rand_seeds
 
     lib = oracle()
-    cores = min(16, os.cpu_count() or 1)
+    cores, visible = host_threads()
     lib.oracle_setNumThreads(cores)
     rng = np.random.default_rng(7)
     N, n = 1 << log2n, 32
@@ -114,14 +134,86 @@ def cpu_baseline(log2n: int):
         times.append(time.perf_counter() - t0)
         assert rc == 0
     t = sorted(times)[1]
-    return {
+    res = {
         "value": N / t,
         "unit": "evals/s",
         "cores": cores,
         "kind": "port",
         "sample": f"DCF n=32 groupSize 1, 2^{log2n} elements, party 0, median of 3 runs, "
                   f"AES-NI={'yes' if lib.oracle_aes_uses_aesni() else 'no'}",
+        "per_core_value": N / t / cores,
+        "host_cpus_visible": visible,
+        "threads_note": "threads = the CPU share given to this process (OMP_NUM_THREADS, else its affinity "
+                        "mask); host_cpus_visible is what the host reports",
     }
+    # the reference's CPU-sized workload in full: n = m = 16 stored in 8-byte elements, 2^16
+    # elements, beta = NULL (= 1), both parties' evals per run
+    N1, n1, es1 = 1 << 16, 16, 8
+    alpha = rand_elems(rng, N1, n1, es1)
+    x1 = rand_elems(rng, N1, n1, es1)
+    t0_, t1_ = rand_seeds(rng, N1), rand_seeds(rng, N1)
+    key1 = dcf_keygen(OracleBackend(), alpha, None, t0_, t1_, n1, n1, 1, es1)
+    o0, o1 = np.zeros(N1, dtype=np.uint64), np.zeros(N1, dtype=np.uint64)
+
+    def both():
+        for party, sd, o in ((0, t0_, o0), (1, t1_, o1)):
+            assert ocall("dcfEval", o, o.nbytes, x1, x1.nbytes, key1, key1.nbytes, sd, sd.nbytes, party, n1, n1,
+                         1, es1, N1, None, 0) == 0
+
+    both()
+    times = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        both()
+        times.append(time.perf_counter() - t0)
+    t = sorted(times)[2]
+    rec_ok = bool(np.array_equal((o0 + o1) & np.uint64(0xFFFF), (x1 < alpha).astype(np.uint64)))
+    res["cpu_sized_workload"] = {
+        "workload": "DCF bitWidthIn=bitWidthOut=16 in 8-byte elements, 2^16 elements, beta=NULL, "
+                    "both parties' evals",
+        "value": 2 * N1 / t, "unit": "evals/s", "ms_both_parties": t * 1e3, "cores": cores,
+        "reconstruction_ok": rec_ok, "sample": "the whole workload, median of 5 runs",
+    }
+    return res
+
+
+def cpu_sized_workload_gpu(reps: int = 20):
+    """The reference's CPU-sized workload (DCF n = m = 16 in 8-byte elements, 2^16 elements,
+    beta = NULL) on the GPU: both parties' evals per step, keys resident."""
+    import torch
+
+    from fastfunctionsecretsharing_amd import _lib
+
+    lib = _lib.load()
+    dev = torch.device("cuda:0")
+    N, n, es = 1 << 16, 16, 8
+    gen = torch.Generator(device=dev).manual_seed(16)
+    alpha = torch.randint(0, 1 << n, (N,), dtype=torch.int64, device=dev, generator=gen)
+    x = torch.randint(0, 1 << n, (N,), dtype=torch.int64, device=dev, generator=gen)
+    seeds = torch.randint(0, 256, (2, 16 * N), dtype=torch.uint8, device=dev, generator=gen)
+    holder = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    sp = ctypes.addressof(holder)
+    ks = ctypes.c_size_t(0)
+    assert lib.FastFss_dcfGetKeyDataSize(ctypes.addressof(ks), n, n, 1, es, N) == 0
+    key = torch.empty(ks.value, dtype=torch.uint8, device=dev)
+    assert lib.FastFss_cuda_dcfKeyGen(key.data_ptr(), ks.value, alpha.data_ptr(), es * N, None, 0,
+                                      seeds[0].data_ptr(), 16 * N, seeds[1].data_ptr(), 16 * N, n, n, 1, es, N,
+                                      sp) == 0
+    outs = [torch.empty(N, dtype=torch.int64, device=dev) for _ in range(2)]
+
+    def both():
+        for party in (0, 1):
+            assert lib.FastFss_cuda_dcfEval(outs[party].data_ptr(), es * N, x.data_ptr(), es * N, key.data_ptr(),
+                                            ks.value, seeds[party].data_ptr(), 16 * N, party, n, n, 1, es, N, None,
+                                            0, sp) == 0
+
+    both()
+    torch.cuda.synchronize()
+    ok = bool(torch.equal((outs[0] + outs[1]) & 0xFFFF, (x < alpha).to(torch.int64)))
+    wall, dev_ms = _timed(torch.cuda.current_stream(dev), both, reps)
+    return {"workload": "DCF bitWidthIn=bitWidthOut=16 in 8-byte elements, 2^16 elements, beta=NULL, "
+                        "both parties' evals", "evals_per_s": 2 * N / wall, "ms_both_parties": wall * 1e3,
+            "device_ms_both_parties": dev_ms, "reconstruction_ok": ok}
 
 
 # per-unit costs: (key schedule, block) in VALU-model ops and in AES table lookups
@@ -247,13 +339,23 @@ def dpf_leg(log2n: int, reps: int = 10):
                                       g.key_size, g.seeds[0].data_ptr(), 16 * N, 0, n, n, 1, 4, N, None, 0, sp)
         assert rc == 0, rc
 
+    # correctness guard: every fifth point is alpha; both parties reconstruct [x == alpha] (beta = 1)
+    g.masked[::5] = g.alpha[::5]
+    step()
+    y0 = out.clone()
+    rc = lib.FastFss_cuda_dpfEval(out.data_ptr(), 4 * N, g.masked.data_ptr(), 4 * N, g.key.data_ptr(), g.key_size,
+                                  g.seeds[1].data_ptr(), 16 * N, 1, n, n, 1, 4, N, None, 0, sp)
+    assert rc == 0, rc
+    rec = (y0.to(torch.int64) + out.to(torch.int64)) & 0xFFFFFFFF
+    ok = bool(torch.equal(rec, (g.masked == g.alpha).to(torch.int64)))
+    del y0, rec
     step()
     wall, dev_ms = _timed(torch.cuda.current_stream(), step, reps)
     ops = (n + 1) * DPF_OPS_PER_LEVEL  # n levels + the value conversion's AES (bitWidthOut > 16)
     achieved = ops * N / (dev_ms * 1e-3) / 1e12
     del holder
     return {"workload": "plain DPF eval, bitWidthIn=32, bitWidthOut=32, groupSize=1, 2^%d elements, party 0" % log2n,
-            "evals_per_s": N / wall, "kernel_ms": dev_ms, "ops_per_eval": ops, "valu_tops": achieved,
+            "parity_ok": ok, "evals_per_s": N / wall, "kernel_ms": dev_ms, "ops_per_eval": ops, "valu_tops": achieved,
             "valu_frac": achieved / VALU_PEAK_TOPS,
             "lds_frac": (n + 1) * sum(LDS_COST) * N / (dev_ms * 1e-3) / 1e9 / LDS_PEAK_GLOOKUPS}
 
@@ -508,17 +610,22 @@ def main():
     }
     del key, out
     torch.cuda.empty_cache()
+    progress("dcf: %.3f ms/step" % res["ms_per_step"])
     if not a.no_dpf:
         res["dpf"] = dpf_leg(a.log2n)
+        progress("dpf done")
         torch.cuda.empty_cache()
     if not a.no_piecewise:
         res["piecewise"] = piecewise_leg()
+        progress("piecewise done")
         torch.cuda.empty_cache()
     if not a.no_ragged:
         res["ragged"] = ragged_leg(a.ragged_groups)
+        progress("ragged done")
     if not a.no_serve:
         torch.cuda.empty_cache()
         res["serving"] = serving_leg(a.serve_qps, a.serve_requests)
+        progress("serving done")
     if a.calib_copy_mib:
         src = torch.empty(a.calib_copy_mib << 20, dtype=torch.uint8, device=dev).fill_(1)
         dst = torch.empty_like(src)
@@ -527,7 +634,10 @@ def main():
         torch.cuda.synchronize()
         res["calib_copy_bytes"] = a.calib_copy_mib << 20
     if not a.no_cpu:
+        res["cpu_sized_workload_gpu"] = cpu_sized_workload_gpu()
+        progress("cpu_sized_workload_gpu done")
         res["cpu_baseline"] = cpu_baseline(a.cpu_sample_log2)
+        progress("cpu_baseline done")
     print(json.dumps(res))
     return 0 if ok else 1
 
