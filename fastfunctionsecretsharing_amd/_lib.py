@@ -81,6 +81,7 @@ SIGNATURES = {
     "FastFss_hip_setEvalVariant": ("i", "int"),
     "FastFss_hip_getEvalVariantCount": ("", "int"),
     "FastFss_hip_setEvalVariantForElementSize": ("si", "int"),
+    "FastFss_hip_dcfEvalNoAlloc": (_EVAL + "p", "int"),
     "FastFss_hip_grottoEvalFullDomain": ("pspsps" + "iii" + "sss" + "p", "int"),
     "FastFss_hip_grottoFullSetLaneTarget": ("s", "int"),
     "FastFss_hip_serverCreate": ("psiii", "int"),

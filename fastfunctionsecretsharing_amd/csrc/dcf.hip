@@ -7,8 +7,10 @@
 // follows the reference Task::check order (src/kernel/dcf.h:31-69, 115-163) so
 // callers see the same error codes.
 #include <atomic>
+#include <mutex>
 
 #include "FastFss/cuda/dcf.h"
+#include "FastFss/hip/no_alloc.h"
 #include "dcf_device.h"
 #include "fss_host.h"
 #include "fss_launch.h"
@@ -296,13 +298,20 @@ int dcf_eval_variant(size_t es, size_t n, size_t N)
     return 2 * N <= (size_t)device_cu_count() * 512 ? 7 : 2;
 }
 
+// `fixed_counter`: a caller-owned tile counter for the work-queue variants
+// (cleared here by a kernel before use), or nullptr for a stream-ordered
+// allocation made and freed around the launch.
 template <typename T>
 int launch_dcf_eval_g1(int v, int grid, int block, hipStream_t st, T *out, const T *x, const uint8_t *key,
-                       const uint8_t *seed, int party, int n, int outBytes, size_t N)
+                       const uint8_t *seed, int party, int n, int outBytes, size_t N, uint32_t *fixed_counter = nullptr)
 {
     if (v == 5 || v == 6)
     {
-        uint32_t *counter = queue_counter_acquire(st);
+        uint32_t *counter = fixed_counter;
+        if (counter)
+            queue_reset_kernel<<<1, 1, 0, st>>>(counter);
+        else
+            counter = queue_counter_acquire(st);
         if (!counter) return FAST_FSS_RUNTIME_ERROR;
         const bool small = block < kBlock;
         if (v == 6 && sizeof(T) <= 4 && small)
@@ -317,7 +326,7 @@ int launch_dcf_eval_g1(int v, int grid, int block, hipStream_t st, T *out, const
         else
             dcf_eval_queue_kernel<T, 8, false>
                 <<<grid, block, 0, st>>>(out, x, key, seed, party, n, outBytes, N, counter);
-        queue_counter_release(st, counter);
+        if (!fixed_counter) queue_counter_release(st, counter);
         return FAST_FSS_SUCCESS;
     }
     if (v == 7)
@@ -348,6 +357,46 @@ int launch_dcf_eval_g1_rt(size_t es, int grid, hipStream_t st, void *out, const 
     FSS_DISPATCH(es, rc = launch_dcf_eval_g1<T>(v, grid, block, st, (T *)out, (const T *)x, (const uint8_t *)key,
                                                 (const uint8_t *)seed, party, n, (m + 7) / 8, N));
     return rc != FAST_FSS_SUCCESS ? rc : launch_status();
+}
+
+// ------------------------------------------ no-allocation DCF evaluation ----
+//
+// FastFss_hip_dcfEvalNoAlloc keeps the work queue's tile counter in this
+// file-scope array instead of a stream-ordered allocation.  Each GPU holds its
+// own copy at its own address until the process ends; host code reaches it
+// only through the address hipGetSymbolAddress returns for that device, kept
+// in g_scratch_ptr.  Every call on one GPU shares the array, so calls that
+// overlap on one GPU race.
+constexpr size_t kNoAllocScratchBytes = 256;
+// one 32-bit tile counter: the tiles (64 elements each) plus one final fetch
+// per wave must fit in it, which holds up to 2^36 elements
+constexpr size_t kNoAllocMaxElems = FAST_FSS_HIP_NO_ALLOC_MAX_ELEMENTS;
+static __device__ __align__(256) uint32_t scratch[kNoAllocScratchBytes / sizeof(uint32_t)];
+
+constexpr int                     kMaxDevices = 64;
+static std::atomic<uint32_t *>    g_scratch_ptr[kMaxDevices];
+static std::mutex                 g_scratch_mu;
+
+// The scratch address on the device that owns `st`; nullptr on failure.
+static uint32_t *noalloc_scratch(hipStream_t st)
+{
+    int dev = -1;
+    if (hipStreamGetDevice(st, &dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+    uint32_t *p = g_scratch_ptr[dev].load(std::memory_order_acquire);
+    if (p) return p;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    p = g_scratch_ptr[dev].load(std::memory_order_relaxed);
+    if (p) return p;
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess) return nullptr;
+    if (cur != dev && hipSetDevice(dev) != hipSuccess) return nullptr;
+    void      *addr = nullptr;
+    hipError_t e    = hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch));
+    if (cur != dev) (void)hipSetDevice(cur);
+    if (e != hipSuccess || !addr) return nullptr;
+    p = (uint32_t *)addr;
+    g_scratch_ptr[dev].store(p, std::memory_order_release);
+    return p;
 }
 
 } // namespace fss
@@ -446,6 +495,51 @@ extern "C" FAST_FSS_API int FastFss_cuda_dcfEval(void *sharedOut, size_t sharedO
         FSS_DISPATCH(es, rc = launch_dcf_eval_g1<T>(v, ls.grid, ls.block, st, (T *)sharedOut, (const T *)maskedX,
                                                     (const uint8_t *)key, (const uint8_t *)seed, partyId, (int)n,
                                                     (int)((m + 7) / 8), N));
+        if (rc != FAST_FSS_SUCCESS) return rc;
+    }
+    else
+    {
+        FSS_DISPATCH(es, dcf_eval_group_kernel<T><<<ls.grid, ls.block, 0, st>>>(
+                             (T *)sharedOut, (const T *)maskedX, (const uint8_t *)key, (const uint8_t *)seed, partyId,
+                             (int)n, (int)m, (int)g, N));
+    }
+    return launch_status();
+}
+
+extern "C" FAST_FSS_API int FastFss_hip_dcfEvalNoAlloc(void *sharedOut, size_t sharedOutSize, const void *maskedX,
+                                                       size_t maskedXDataSize, const void *key, size_t keyDataSize,
+                                                       const void *seed, size_t seedDataSize, int partyId,
+                                                       size_t bitWidthIn, size_t bitWidthOut, size_t groupSize,
+                                                       size_t elementSize, size_t elementNum, void *cache,
+                                                       size_t cacheDataSize, void *hipStreamPtr)
+{
+    (void)cache;
+    const size_t n = bitWidthIn, m = bitWidthOut, g = groupSize, es = elementSize, N = elementNum;
+    if (!elem_size_ok(es)) return FAST_FSS_INVALID_ELEMENT_SIZE_ERROR;
+    if (!(0 < n && n <= es * 8)) return FAST_FSS_INVALID_BITWIDTH_ERROR;
+    if (!(0 < m && m <= es * 8)) return FAST_FSS_INVALID_BITWIDTH_ERROR;
+    if (partyId != 0 && partyId != 1) return FAST_FSS_INVALID_PARTY_ID_ERROR;
+    if (g == 0) return FAST_FSS_INVALID_GROUP_SIZE_ERROR;
+    if (keyDataSize != dcf_key_size(n, g, es, N)) return FAST_FSS_INVALID_KEY_DATA_SIZE_ERROR;
+    if (sharedOutSize != N * es * g) return FAST_FSS_INVALID_SHARED_OUT_DATA_SIZE_ERROR;
+    if (maskedXDataSize != N * es) return FAST_FSS_INVALID_MASKED_X_DATA_SIZE_ERROR;
+    if (seedDataSize != N * 16) return FAST_FSS_INVALID_SEED_DATA_SIZE_ERROR;
+    if (cacheDataSize != 0 && cacheDataSize != dcf_cache_size(n, g, es, N)) return FAST_FSS_INVALID_CACHE_DATA_SIZE_ERROR;
+    // beyond this the scratch's 32-bit tile counter could wrap
+    if (N > kNoAllocMaxElems) return FAST_FSS_RUNTIME_ERROR;
+    if (N == 0) return FAST_FSS_SUCCESS;
+    if (!aligned(key, 8) || !aligned(seed, 8)) return FAST_FSS_RUNTIME_ERROR;
+    hipStream_t       st = resolve_stream(hipStreamPtr);
+    const LaunchShape ls = shape_for(N);
+    if (g == 1)
+    {
+        uint32_t *counter = noalloc_scratch(st);
+        if (!counter) return FAST_FSS_RUNTIME_ERROR;
+        const int v  = dcf_eval_variant(es, n, N);
+        int       rc = FAST_FSS_SUCCESS;
+        FSS_DISPATCH(es, rc = launch_dcf_eval_g1<T>(v, ls.grid, ls.block, st, (T *)sharedOut, (const T *)maskedX,
+                                                    (const uint8_t *)key, (const uint8_t *)seed, partyId, (int)n,
+                                                    (int)((m + 7) / 8), N, counter));
         if (rc != FAST_FSS_SUCCESS) return rc;
     }
     else
