@@ -419,6 +419,40 @@ def ottt_eval(be, x, key, party, lut, n, es):
     return oe, ot
 
 
+def sub_key_rows(key, idx, sk, tail, N):
+    """The key bytes of elements idx from a split key [first part: N x sk][tail: N x tail], in the same
+    layout for len(idx) elements (DCF: sk = 16n, tail = (n g + 2 + g) es; DPF: sk = 16n, tail = (2 + g) es;
+    Grotto: three parts, not covered)."""
+    key = np.asarray(key)
+    return np.concatenate([key[:N * sk].reshape(N, sk)[idx].ravel(), key[N * sk:N * (sk + tail)].reshape(N, tail)[idx].ravel()])
+
+
+def spline_closed_form(X, alpha, left, right, coeff, d, n, m):
+    """What two spline shares reconstruct to, from the mathematics alone (not the oracle): for the
+    interval q holding x = (X - alpha) mod 2^n (unsigned, sorted disjoint intervals), f_q evaluated
+    at the integer X - alpha, mod 2^m; 0 where no interval holds x. Evaluated in uint64 arithmetic,
+    exact mod 2^m because 2^m divides 2^64."""
+    X = np.asarray(X).astype(np.uint64)
+    A = np.asarray(alpha).astype(np.uint64)
+    x = (X - A) & np.uint64(mask(n))
+    L = np.asarray(left).astype(np.uint64)
+    R = np.asarray(right).astype(np.uint64)
+    order = np.argsort(L)
+    q = np.searchsorted(L[order], x, side="right") - 1
+    hit = q >= 0
+    qi = order[np.clip(q, 0, None)]
+    hit &= x <= R[qi]
+    C = np.asarray(coeff).astype(np.uint64).reshape(-1, d + 1)
+    t = X - A  # the integer X - alpha, two's complement in 64 bits
+    acc = np.zeros_like(X)
+    pw = np.ones_like(X)
+    with np.errstate(over="ignore"):
+        for k in range(d + 1):
+            acc += C[qi, k] * pw
+            pw *= t
+    return np.where(hit, acc & np.uint64(mask(m)), np.uint64(0))
+
+
 def signed_nbit(v: int, bits: int) -> int:
     v &= mask(bits)
     return v - (1 << bits) if (bits and v >> (bits - 1)) else v

@@ -10,7 +10,7 @@ import ctypes
 import numpy as np
 import pytest
 
-from fss_testlib import OracleBackend, dcf_eval, dpf_eval
+from fss_testlib import DT, OracleBackend, dcf_eval, dpf_eval, sub_key_rows
 
 pytestmark = pytest.mark.gpu
 
@@ -115,4 +115,55 @@ def test_batch_validation(setup):
     desc[0].elementSize = 3
     assert lib.FastFss_hip_batchEval(desc, 2, sp) == -11
     assert lib.FastFss_hip_batchEval(desc, 0, sp) == 0
+    del holder
+
+
+def test_ragged_batch_1024_groups(setup):
+    """The ragged workload at its stated size: the benchmark's 1024 groups (log-uniform 2^6..2^18
+    elements, bitWidthIn = bitWidthOut in {8, 16, 24, 32, 64}, DCF:DPF 70:30, 31M elements) in one
+    captured plan, replayed for each party. Every group equals the per-call entry point, every
+    group reconstructs, and 128 elements of one group per (op, width) equal the oracle's shares."""
+    torch, _lib, W, _ = setup
+    lib = _lib.load()
+    groups = W.ragged_batch(groups=1024, seed=11)
+    assert len(groups) == 1024 and sum(g.N for g in groups) > 30_000_000
+    stream = torch.cuda.Stream()
+    holder = ctypes.c_void_p(stream.cuda_stream)
+    sp = ctypes.addressof(holder)
+    outs = {}
+    for party in (0, 1):
+        o = W.alloc_outputs(groups)
+        plan = ctypes.c_void_p()
+        assert lib.FastFss_hip_batchPlanCreate(ctypes.byref(plan), W.batch_descriptors(groups, party, o),
+                                               len(groups)) == 0
+        assert lib.FastFss_hip_batchPlanCapture(plan, sp) == 0
+        assert lib.FastFss_hip_batchPlanReplay(plan, sp) == 0
+        stream.synchronize()
+        lib.FastFss_hip_batchPlanDestroy(plan)
+        ref = W.alloc_outputs(groups)
+        with torch.cuda.stream(stream):
+            W.eval_plain(groups, party, ref, sp)
+        stream.synchronize()
+        for i, (a, b) in enumerate(zip(o, ref)):
+            assert torch.equal(a, b), (party, i, groups[i].op, groups[i].n, groups[i].N)
+        del ref
+        outs[party] = o
+        print(f"party {party}: 1024 batched groups equal the per-call launches", flush=True)
+    assert W.reconstruct_ok(groups, outs[0], outs[1])
+    OR = OracleBackend()
+    seen = set()
+    for i, g in enumerate(groups):
+        if (g.op, g.n) in seen or g.N < 256:
+            continue
+        seen.add((g.op, g.n))
+        es, n, N = g.es, g.n, g.N
+        idx = np.unique(np.concatenate([np.random.default_rng(i).choice(N, 126, replace=False), [0, N - 1]]))
+        sub = sub_key_rows(g.key.cpu().numpy(), idx, 16 * n, (n + 3) * es if g.op == _lib.BATCH_DCF_EVAL else 3 * es, N)
+        x = g.masked.view(torch.uint8).cpu().numpy().view(DT[es])[idx]
+        fn = dcf_eval if g.op == _lib.BATCH_DCF_EVAL else dpf_eval
+        for party in (0, 1):
+            seed = g.seeds[party].cpu().numpy().reshape(N, 16)[idx].ravel()
+            got = outs[party][i].view(torch.uint8).cpu().numpy().view(DT[es])[idx]
+            assert np.array_equal(got, fn(OR, x, sub, seed, party, n, n, 1, es)), (i, party, g.op, n)
+    assert len(seen) == 10, seen
     del holder

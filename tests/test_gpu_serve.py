@@ -4,6 +4,8 @@ evaluated alone through the per-call entry point."""
 import numpy as np
 import pytest
 
+from fss_testlib import DT, OracleBackend, dcf_eval, mic_eval, spline_eval, sub_key_rows
+
 pytestmark = pytest.mark.gpu
 
 
@@ -135,3 +137,63 @@ def test_concurrent_submitters(env):
         ref = _alone(torch, W, templates, picks[k])
         for i in range(10):
             assert torch.equal(outs[k][i], ref[i]), (k, i)
+
+
+def _np(torch, t, es):
+    return t.view(torch.uint8).cpu().numpy().view(DT[es])
+
+
+def test_serving_bench_trace(env):
+    """The serving workload at its stated sizes: the 48 resident key sets and the 400-request
+    Poisson trace (400 req/s) the benchmark serves, DCF/MIC/spline 1:1:1, 2^10..2^20 elements,
+    bitWidthIn in {16, 32, 64}, under the benchmark's served configuration. Every request equals
+    its alone result, and 64 elements of every key set's served output equal the oracle's."""
+    torch, _lib, W, _ = env
+    templates = W.serving_templates(count=48, seed=5)
+    rng = np.random.default_rng(17)
+    picks = [int(p) for p in rng.integers(0, len(templates), size=400)]
+    arrivals = np.cumsum(rng.exponential(1.0 / 400, size=400))
+    sizes = [templates[p].N for p in picks]
+    assert min(sizes) >= 1 << 10 and max(sizes) <= 1 << 20 and max(sizes) > 1 << 19
+    outs, lats, wall, st = W.run_trace(templates, picks, arrivals, 32 << 30, 16, 8, 1)
+    assert st.completed == len(picks) and st.failed == 0
+    print(f"served 400 requests in {wall:.2f} s, max in flight {st.maxInflightSeen}", flush=True)
+    ref = _alone(torch, W, templates, picks)
+    for i, (a, b) in enumerate(zip(outs, ref)):
+        assert torch.equal(a, b), (i, templates[picks[i]].op, templates[picks[i]].n, templates[picks[i]].N)
+    OR = OracleBackend()
+    first = {}
+    for i, p in enumerate(picks):
+        first.setdefault(p, i)
+    checked = set()
+    for p, i in sorted(first.items()):
+        t = templates[p]
+        es, n, N = t.es, t.n, t.N
+        idx = np.unique(np.concatenate([np.random.default_rng(p).choice(N, 62, replace=False), [0, N - 1]]))
+        key = t.key.cpu().numpy()
+        x = _np(torch, t.masked, es)[idx]
+        seed = t.seed0.cpu().numpy().reshape(N, 16)[idx].ravel()
+        got = _np(torch, outs[i], es)
+        if t.op == _lib.SERVE_DCF:
+            sub = sub_key_rows(key, idx, 16 * n, (n + 3) * es, N)
+            want = dcf_eval(OR, x, sub, seed, 0, n, n, 1, es)
+            assert np.array_equal(got[idx], want), ("dcf", p)
+        elif t.op == _lib.SERVE_MIC:
+            I = t.intervals
+            sub = sub_key_rows(key, idx, 16 * n, (n + 3) * es, N)
+            z = _np(torch, t.z, es).reshape(N, I)[idx].ravel()
+            lf, rt = _np(torch, t.left, es), _np(torch, t.right, es)
+            want = mic_eval(OR, x, sub, z, seed, 0, lf, rt, n, n, es)
+            assert np.array_equal(got.reshape(N, I)[idx].ravel(), want), ("mic", p)
+        else:
+            I, d = t.intervals, t.degree
+            g = I * (d + 1)
+            sub = sub_key_rows(key, idx, 16 * n, (n * g + 2 + g) * es, N)
+            e = _np(torch, t.z, es).reshape(N, g)[idx].ravel()
+            b = _np(torch, t.beta, es).reshape(N, g)[idx].ravel()
+            lf, rt = _np(torch, t.left, es), _np(torch, t.right, es)
+            want = spline_eval(OR, x, sub, e, b, seed, 0, lf, rt, d, n, n, es)
+            assert np.array_equal(got[idx], want), ("spline", p)
+        checked.add((t.op, n))
+    assert {op for op, _ in checked} == {_lib.SERVE_DCF, _lib.SERVE_MIC, _lib.SERVE_SPLINE}
+    assert {n for _, n in checked} == {16, 32, 64}
