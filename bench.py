@@ -88,6 +88,9 @@ def parse():
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--no-dpf", action="store_true", help="skip the plain DPF leg")
     ap.add_argument("--no-piecewise", action="store_true", help="skip the MIC + spline + full-domain Grotto leg")
+    ap.add_argument("--no-ottt", action="store_true", help="skip the OTTT leg")
+    ap.add_argument("--only", default="", help="comma-separated legs to run after the headline step "
+                    "(dpf, piecewise, ottt, ragged, serving, cpu); default: all not skipped")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after timing, run one device-to-device copy of this many MiB (counter calibration)")
     return ap.parse_args()
@@ -360,6 +363,70 @@ def dpf_leg(log2n: int, reps: int = 10):
             "lds_frac": (n + 1) * sum(LDS_COST) * N / (dev_ms * 1e-3) / 1e9 / LDS_PEAK_GLOOKUPS}
 
 
+def ottt_leg(reps: int = 10):
+    """OTTT LUT eval (32-bit elements) at three table sizes: n = 8 with 2 tables over 2^22 elements,
+    n = 12 over 2^20 and n = 16 over 2^16 (its table staged in LDS chunks). Party 0 is timed; both
+    parties' E*T reconstruct LUT[x] on every element. Algorithmic HBM bytes per element: the key
+    (2^n / 8), x and the 1 + lutNum outputs; the tables are read from LDS, 2^n * lutNum entries per
+    element (one per key bit and table)."""
+    import torch
+
+    from fastfunctionsecretsharing_amd import _lib
+
+    lib = _lib.load()
+    dev = torch.device("cuda:0")
+    holder = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    sp = ctypes.addressof(holder)
+    res = {}
+    M32 = 0xFFFFFFFF
+    for n, lutNum, log2N in ((8, 2, 22), (12, 1, 20), (16, 1, 16)):
+        N, es, total = 1 << log2N, 4, 1 << n
+        gen = torch.Generator(device=dev).manual_seed(n)
+        alpha = torch.randint(0, total, (N,), dtype=torch.int64, device=dev, generator=gen)
+        x = torch.randint(0, total, (N,), dtype=torch.int64, device=dev, generator=gen)
+        masked = ((x + alpha) & (total - 1)).to(torch.int32)
+        key0 = torch.randint(0, 256, (N * total // 8,), dtype=torch.uint8, device=dev, generator=gen)
+        key1 = key0.clone()
+        lut = torch.randint(0, 1 << 32, (lutNum * total,), dtype=torch.int64, device=dev, generator=gen)
+        lut32 = lut.to(torch.int32)
+        a32 = alpha.to(torch.int32)
+        assert lib.FastFss_cuda_otttKeyGen(key1.data_ptr(), key1.numel(), a32.data_ptr(), 4 * N, n, es, N, sp) == 0
+        E = [torch.empty(N, dtype=torch.int32, device=dev) for _ in range(2)]
+        T = [torch.empty(N * lutNum, dtype=torch.int32, device=dev) for _ in range(2)]
+
+        def step(party=0):
+            k = key0 if party == 0 else key1
+            rc = lib.FastFss_cuda_otttLutEval(E[party].data_ptr(), 4 * N, T[party].data_ptr(), 4 * N * lutNum,
+                                              masked.data_ptr(), 4 * N, k.data_ptr(), k.numel(), party,
+                                              lut32.data_ptr(), 4 * lutNum * total, n, es, N, sp)
+            assert rc == 0, rc
+
+        step(0)
+        step(1)
+        e = (E[0].to(torch.int64) + E[1].to(torch.int64)) & M32
+        t = ((T[0].to(torch.int64) + T[1].to(torch.int64)) & M32).view(N, lutNum)
+        got = torch.where((e == 1)[:, None], t, (-t) & M32)
+        want = torch.stack([lut[j * total + x] for j in range(lutNum)], dim=1)
+        ok = bool(((e == 1) | (e == M32)).all()) and bool(torch.equal(got, want))
+        del e, t, got, want
+        wall, dev_ms = _timed(torch.cuda.current_stream(dev), step, reps)
+        hbm_bytes = N * (total // 8 + es + es * (1 + lutNum))
+        lookups = N * total * lutNum
+        res["n%d" % n] = {
+            "workload": "OTTT LUT eval, bitWidthIn=%d, %d table(s) of 32-bit entries, 2^%d elements, party 0"
+                        % (n, lutNum, log2N),
+            "parity_ok": ok, "elements_per_s": N / wall, "kernel_ms": dev_ms,
+            "hbm_bytes_per_element": hbm_bytes // N, "hbm_gbs": hbm_bytes / (dev_ms * 1e-3) / 1e9,
+            "hbm_frac": hbm_bytes / (dev_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "lds_lookups_per_element": total * lutNum,
+            "lds_frac": lookups / (dev_ms * 1e-3) / 1e9 / LDS_PEAK_GLOOKUPS,
+        }
+        del key0, key1, E, T, lut, lut32
+        torch.cuda.empty_cache()
+    del holder
+    return res
+
+
 def spline_ops(w, cost=VALU_COST):
     """Algorithmic VALU ops of one spline element: the range walks of the
     interval sweep (one per interval end, plus one per interval start that does
@@ -500,6 +567,10 @@ def serving_leg(qps: float, requests: int, sweep_qps=(200, 800, "burst")):
 
 def main():
     a = parse()
+    if a.only:
+        legs = set(a.only.split(","))
+        a.no_dpf, a.no_piecewise, a.no_ottt = "dpf" not in legs, "piecewise" not in legs, "ottt" not in legs
+        a.no_ragged, a.no_serve, a.no_cpu = "ragged" not in legs, "serving" not in legs, "cpu" not in legs
     if a.gpus != 1:
         print(json.dumps({"error": "this benchmark measures one GPU; multi-GPU runs are not supported",
                           "n_gpus": a.gpus}))
@@ -619,6 +690,9 @@ def main():
         res["piecewise"] = piecewise_leg()
         progress("piecewise done")
         torch.cuda.empty_cache()
+    if not a.no_ottt:
+        res["ottt"] = ottt_leg()
+        progress("ottt done")
     if not a.no_ragged:
         res["ragged"] = ragged_leg(a.ragged_groups)
         progress("ragged done")

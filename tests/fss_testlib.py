@@ -480,6 +480,8 @@ class DeviceBackend:
         if hasattr(a, "data_ptr"):
             return a.contiguous()
         a = np.ascontiguousarray(a)
+        if a.size == 0:
+            return self.torch.empty(0, dtype=_TORCH_DT(self.torch)[a.dtype.itemsize], device=self.dev)
         t = self.torch.from_numpy(a.view(np.uint8).copy()).to(self.dev)
         return t.view(_TORCH_DT(self.torch)[a.dtype.itemsize]) if a.dtype != np.uint8 else t
 
