@@ -28,44 +28,59 @@ REPO = Path(__file__).resolve().parent
 # MI355X: 256 CUs x 64 int32 lane-ops/clk x 2.4 GHz (published peak clock)
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
 HBM_PEAK_GBS = 8000.0
-# Algorithmic cost of one DCF eval (groupSize 1, taken child only) at n levels:
-# per level one AES-128 key schedule (10 rounds x 12 int ops) and two blocks
-# (10 rounds x 32 int ops: 16 byte extracts + 16 XORs on T-table lookups).
-OPS_PER_LEVEL = 10 * 12 + 2 * 10 * 32
-# DPF / Grotto per-point level: one key schedule and one block.
-DPF_OPS_PER_LEVEL = 10 * 12 + 10 * 32
 # HBM bytes per eval at n=32, u32: key 652 (sCW 512 + vCW 128 + tLCW, tRCW,
 # lastCW 12) + seed 16 + masked x 4 + share out 4.
 BYTES_PER_EVAL_N32 = 652 + 16 + 4 + 4
 # HBM bytes per launch of the eval kernel at 2^24 elements, from the two
 # calibrated rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) recorded here
 TRAFFIC_PROFILE = REPO / "profiles" / "dcf_eval_queue_pre_n32_2p24_traffic.md"
-# LDS table lookups per eval of the same kernel (rocprofv3 SQ_INSTS_LDS x 64 /
-# evals, recorded here): the pipe that binds the T-table AES
+# LDS and VALU instructions per eval of the same kernel (rocprofv3 SQ_INSTS_LDS
+# and SQ_INSTS_VALU x 64 / evals, recorded here)
 TRACE_PROFILE = REPO / "profiles" / "dcf_eval_queue_pre_n32_2p24_kernel_trace.md"
-# MI355X LDS: 128 B/clk/CU = 32 four-byte lookups per clock per CU
-LDS_PEAK_GLOOKUPS = 256 * 32 * 2.4e9 / 1e9
+# The LDS read ceiling for 4-byte lookups measured by the LDS probe
+# (probe/lds_probe.hip), recorded here; bench.py also measures it live on the
+# box it runs on and takes the live figure as the roofline peak
+LDS_PEAK_PROFILE = REPO / "profiles" / "lds_peak.md"
+# probe layouts whose reads are 4-byte lookups (one or two per instruction)
+LDS_LOOKUP_LAYOUTS = (0, 1, 5, 7)
 # Algorithmic AES table lookups per DCF level (taken child only): one key
 # schedule (10 rounds x 4 S-box lookups) and two blocks (10 rounds x 16)
 LOOKUPS_PER_LEVEL = 10 * 4 + 2 * 10 * 16
 
 
+def profiled_value(path: Path, key: str):
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        if line.startswith(key + ":"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def recorded_lds_peak():
+    """The recorded LDS lookup ceiling in G lookups/s (words per clock per CU x CUs x 2.4 GHz)."""
+    w = profiled_value(LDS_PEAK_PROFILE, "lds_lookup_words_per_clk_per_cu")
+    return w * 256 * 2.4 if w else None
+
+
+# G lookups/s the legs' LDS fractions are taken against: the live probe's figure once main() has
+# measured it, the recorded one before
+LDS_PEAK = {"glookups": recorded_lds_peak(), "source": "recorded (profiles/lds_peak.md)"}
+
+
 def profiled_traffic(log2n: int):
-    if log2n != 24 or not TRAFFIC_PROFILE.exists():
+    if log2n != 24:
         return None
-    for line in TRAFFIC_PROFILE.read_text().splitlines():
-        if line.startswith("traffic_bytes_per_launch:"):
-            return int(line.split(":", 1)[1])
-    return None
+    v = profiled_value(TRAFFIC_PROFILE, "traffic_bytes_per_launch")
+    return int(v) if v else None
 
 
-def profiled_lds_per_eval(log2n: int):
-    if log2n != 24 or not TRACE_PROFILE.exists():
-        return None
-    for line in TRACE_PROFILE.read_text().splitlines():
-        if line.startswith("lds_instructions_per_eval:"):
-            return int(line.split(":", 1)[1])
-    return None
+def profiled_per_eval(log2n: int, key: str):
+    return profiled_value(TRACE_PROFILE, key) if log2n == 24 else None
+
+
+def lds_frac(lookups_per_s: float):
+    return lookups_per_s / 1e9 / LDS_PEAK["glookups"] if LDS_PEAK["glookups"] else None
 
 
 def progress(msg: str):
@@ -219,15 +234,14 @@ def cpu_sized_workload_gpu(reps: int = 20):
             "device_ms_both_parties": dev_ms, "reconstruction_ok": ok}
 
 
-# per-unit costs: (key schedule, block) in VALU-model ops and in AES table lookups
-VALU_COST = (10 * 12, 10 * 32)
+# AES table lookups of one key schedule and of one block
 LDS_COST = (10 * 4, 10 * 16)
 
 
-def ragged_ops(groups, cost=VALU_COST):
-    """Algorithmic work of one pass over a ragged batch: DCF n x (schedule + 2 blocks),
-    DPF n x (schedule + block) plus one more level for the value conversion when
-    bitWidthOut > 16.  `cost` = (key schedule, block): VALU-model ops or lookups."""
+def ragged_ops(groups, cost=LDS_COST):
+    """Algorithmic AES table lookups of one pass over a ragged batch: DCF n x (schedule + 2 blocks),
+    DPF n x (schedule + block) plus one more level for the value conversion when bitWidthOut > 16.
+    `cost` = lookups of (key schedule, block)."""
     from fastfunctionsecretsharing_amd import _lib
 
     ks, blk = cost
@@ -300,9 +314,8 @@ def ragged_leg(groups_n: int, reps: int = 5):
         "batched_device_ms": batch_dev_ms,
         "plain_device_ms": plain_dev_ms,
         "batched_equals_plain": same,
-        "batched_valu_frac": ragged_ops(groups) / (batch_dev_ms * 1e-3) / 1e12 / VALU_PEAK_TOPS,
-        "batched_lds_frac": ragged_ops(groups, LDS_COST) / (batch_dev_ms * 1e-3) / 1e9 / LDS_PEAK_GLOOKUPS,
-        "plain_valu_frac": ragged_ops(groups) / (plain_dev_ms * 1e-3) / 1e12 / VALU_PEAK_TOPS,
+        "batched_lds_frac": lds_frac(ragged_ops(groups) / (batch_dev_ms * 1e-3)),
+        "plain_lds_frac": lds_frac(ragged_ops(groups) / (plain_dev_ms * 1e-3)),
         "groups_per_width": widths,
         "autotuned_tiles": {"%s/es%d" % ("dcf" if op == 0 else "dpf", es): t for (op, es), t in sorted(tiles.items())},
         "autotune_s": tune_s,
@@ -354,13 +367,45 @@ def dpf_leg(log2n: int, reps: int = 10):
     del y0, rec
     step()
     wall, dev_ms = _timed(torch.cuda.current_stream(), step, reps)
-    ops = (n + 1) * DPF_OPS_PER_LEVEL  # n levels + the value conversion's AES (bitWidthOut > 16)
-    achieved = ops * N / (dev_ms * 1e-3) / 1e12
+    lookups = (n + 1) * sum(LDS_COST)  # n levels + the value conversion's AES (bitWidthOut > 16)
     del holder
     return {"workload": "plain DPF eval, bitWidthIn=32, bitWidthOut=32, groupSize=1, 2^%d elements, party 0" % log2n,
-            "parity_ok": ok, "evals_per_s": N / wall, "kernel_ms": dev_ms, "ops_per_eval": ops, "valu_tops": achieved,
-            "valu_frac": achieved / VALU_PEAK_TOPS,
-            "lds_frac": (n + 1) * sum(LDS_COST) * N / (dev_ms * 1e-3) / 1e9 / LDS_PEAK_GLOOKUPS}
+            "parity_ok": ok, "evals_per_s": N / wall, "kernel_ms": dev_ms, "lookups_per_eval": lookups,
+            "lds_frac": lds_frac(lookups * N / (dev_ms * 1e-3))}
+
+
+LDS_PROBE_LAYOUTS = {
+    0: "ds_read_b32, one address register per read, 256-B rows, bank L mod 32 (the AES tables' pattern)",
+    1: "ds_read_b32, one address register per read, 256-B rows, 64 distinct banks",
+    2: "ds_read_b32, shared address register, rows 1028 B apart (straddling 256-B lines), 8 in flight",
+    3: "ds_read_b64, one address register per read, 512-B rows",
+    4: "ds_read_b128, 1024-B rows",
+    5: "ds_read2st64_b32 (two 4-byte reads per instruction), 256-B rows",
+    6: "as 0 with 8 reads in flight per wave instead of 32",
+    7: "ds_read_b32, 32 per asm block at immediate offsets from one address register, 256-B rows, 64 banks",
+}
+
+
+def lds_probe_leg(iters: int = 5000):
+    """The LDS read ceiling, measured: fss_lds_probe (fastfunctionsecretsharing_amd/probe/lds_probe.hip)
+    streams conflict-free reads, 32 independent per wave in flight, on one 1024-lane workgroup per CU,
+    for each read layout; the rate is 4-byte words delivered per clock per CU at the 2.4 GHz clock
+    the published peaks use."""
+    import torch
+
+    lib = ctypes.CDLL(str(REPO / "fastfunctionsecretsharing_amd" / "libfss_lds_probe.so"))
+    lib.fss_lds_probe.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                  ctypes.POINTER(ctypes.c_double)]
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    res = {"blocks": cus, "lanes_per_block": 1024, "iters": iters, "clock_ghz": 2.4, "layouts": {}}
+    for layout, what in LDS_PROBE_LAYOUTS.items():
+        ms, words = ctypes.c_double(), ctypes.c_double()
+        assert lib.fss_lds_probe(layout, cus, iters, ctypes.byref(ms), ctypes.byref(words)) == 0
+        rate = words.value / (ms.value * 1e-3)
+        res["layouts"][str(layout)] = {"reads": what, "kernel_ms": ms.value, "gwords_per_s": rate / 1e9,
+                                      "words_per_clk_per_cu": rate / (cus * 2.4e9),
+                                      "bytes_per_clk_per_cu": 4 * rate / (cus * 2.4e9)}
+    return res
 
 
 def ottt_leg(reps: int = 10):
@@ -419,7 +464,7 @@ def ottt_leg(reps: int = 10):
             "hbm_bytes_per_element": hbm_bytes // N, "hbm_gbs": hbm_bytes / (dev_ms * 1e-3) / 1e9,
             "hbm_frac": hbm_bytes / (dev_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "lds_lookups_per_element": total * lutNum,
-            "lds_frac": lookups / (dev_ms * 1e-3) / 1e9 / LDS_PEAK_GLOOKUPS,
+            "lds_frac": lds_frac(lookups / (dev_ms * 1e-3)),
         }
         del key0, key1, E, T, lut, lut32
         torch.cuda.empty_cache()
@@ -427,8 +472,8 @@ def ottt_leg(reps: int = 10):
     return res
 
 
-def spline_ops(w, cost=VALU_COST):
-    """Algorithmic VALU ops of one spline element: the range walks of the
+def spline_ops(w, cost=LDS_COST):
+    """Algorithmic AES table lookups of one spline element: the range walks of the
     interval sweep (one per interval end, plus one per interval start that does
     not follow its predecessor), each n levels of (key schedule + 2 blocks) for
     the seed and (key schedule + one block per 16-byte window of its slots) for
@@ -472,38 +517,34 @@ def piecewise_leg(reps: int = 3):
     out = torch.empty(N * I, dtype=torch.int32, device="cuda:0")
     W.mic_eval(w, 0, out, w.z, sp)
     wall, dev_ms = _timed(stream, lambda: W.mic_eval(w, 0, out, w.z, sp), reps)
-    ops = (I + 1) * w.n * OPS_PER_LEVEL  # contiguous intervals share endpoints: I + 1 DCF evals per element
+    # contiguous intervals share endpoints: I + 1 DCF walks per element
+    lookups = (I + 1) * w.n * (LDS_COST[0] + 2 * LDS_COST[1])
     res["mic"] = {"workload": "DCF-MIC, 16 contiguous intervals, bitWidthIn=24, bitWidthOut=32, 2^22 elements",
                   "elements_per_s": N / wall, "interval_evals_per_s": N * I / wall, "kernel_ms": dev_ms,
-                  "valu_frac": ops * N / (dev_ms * 1e-3) / 1e12 / VALU_PEAK_TOPS,
-                  "lds_frac": (I + 1) * w.n * (LDS_COST[0] + 2 * LDS_COST[1]) * N / (dev_ms * 1e-3) / 1e9
-                  / LDS_PEAK_GLOOKUPS}
+                  "lookups_per_element": lookups, "lds_frac": lds_frac(lookups * N / (dev_ms * 1e-3))}
     del out
     cache = W.spline_cache(w)
     out = torch.empty(N, dtype=torch.int32, device="cuda:0")
     W.spline_eval(w, 0, out, w.e, w.beta, cache, sp)
     wall, dev_ms = _timed(stream, lambda: W.spline_eval(w, 0, out, w.e, w.beta, cache, sp), reps)
-    ops = N * spline_ops(w)
+    lookups = spline_ops(w)
     res["spline"] = {"workload": "DCF-spline degree 2, 16 contiguous intervals, bitWidthIn=24, bitWidthOut=32, "
                                  "2^22 elements, int32 coefficients",
                      "elements_per_s": N / wall, "kernel_ms": dev_ms, "key_bytes": w.spl_ks,
-                     "ops_per_element": ops // N, "valu_frac": ops / (dev_ms * 1e-3) / 1e12 / VALU_PEAK_TOPS,
-                     "lds_frac": N * spline_ops(w, LDS_COST) / (dev_ms * 1e-3) / 1e9 / LDS_PEAK_GLOOKUPS}
+                     "lookups_per_element": lookups, "lds_frac": lds_frac(N * lookups / (dev_ms * 1e-3))}
     del w, out, cache
     torch.cuda.empty_cache()
     gs = W.grotto_keys()
     lv = gs.n - 6
     pts = gs.N << gs.n
-    ops = gs.N * ((1 << lv) - 1) * OPS_PER_LEVEL  # one key schedule + two blocks per inner node
+    lookups = gs.N * ((1 << lv) - 1) * (LDS_COST[0] + 2 * LDS_COST[1])  # a schedule + two blocks per inner node
     full = {"workload": "full-domain Grotto [x < alpha] shares, bitWidthIn=16, 4096 keys x 2^16 points, party 0"}
     for name, packed, out in (("elements", 0, torch.empty(pts, dtype=torch.int16, device="cuda:0")),
                               ("packed_bits", 1, torch.empty(pts // 64, dtype=torch.int64, device="cuda:0"))):
         W.grotto_full(gs, 0, _lib.GROTTO_FULL_LT, packed, out, sp)
         wall, dev_ms = _timed(stream, lambda: W.grotto_full(gs, 0, _lib.GROTTO_FULL_LT, packed, out, sp), reps * 3)
         full[name] = {"points_per_s": pts / wall, "kernel_ms": dev_ms,
-                      "valu_frac": ops / (dev_ms * 1e-3) / 1e12 / VALU_PEAK_TOPS,
-                      "lds_frac": gs.N * ((1 << lv) - 1) * (LDS_COST[0] + 2 * LDS_COST[1]) / (dev_ms * 1e-3) / 1e9
-                      / LDS_PEAK_GLOOKUPS,
+                      "lds_frac": lds_frac(lookups / (dev_ms * 1e-3)),
                       "out_gbs": out.numel() * out.element_size() / (dev_ms * 1e-3) / 1e9}
         del out
     res["grotto_full_domain"] = full
@@ -629,11 +670,20 @@ def main():
     kernel_ms = ev0.elapsed_time(ev1) / a.steps
     evals_per_s = N * a.steps / wall
 
-    ops = n * OPS_PER_LEVEL * N
-    achieved_tops = ops / (kernel_ms * 1e-3) / 1e12
+    del key, out
+    torch.cuda.empty_cache()
+    # the LDS lookup ceiling on this box, measured right after the timed steps
+    probe = lds_probe_leg()
+    best = max(LDS_LOOKUP_LAYOUTS, key=lambda k: probe["layouts"][str(k)]["words_per_clk_per_cu"])
+    live = probe["layouts"][str(best)]
+    LDS_PEAK["glookups"] = live["gwords_per_s"]
+    LDS_PEAK["source"] = "LDS probe, this run: layout %d (%s)" % (best, live["reads"])
+
     alg_glookups = n * LOOKUPS_PER_LEVEL * N / (kernel_ms * 1e-3) / 1e9
-    lds_per_eval = profiled_lds_per_eval(a.log2n)
+    lds_per_eval = profiled_per_eval(a.log2n, "lds_instructions_per_eval")
+    valu_per_eval = profiled_per_eval(a.log2n, "valu_instructions_per_eval")
     lds_g = lds_per_eval * N / (kernel_ms * 1e-3) / 1e9 if lds_per_eval else None
+    valu_t = valu_per_eval * N / (kernel_ms * 1e-3) / 1e12 if valu_per_eval else None
     hbm_gbs = BYTES_PER_EVAL_N32 * N / (kernel_ms * 1e-3) / 1e9
 
     res = {
@@ -656,31 +706,34 @@ def main():
         "roofline": {
             "bound": "lds",
             "achieved": alg_glookups,
-            "peak": LDS_PEAK_GLOOKUPS,
+            "peak": LDS_PEAK["glookups"],
             "unit": "Glookups/s",
-            "frac": alg_glookups / LDS_PEAK_GLOOKUPS,
-            "lookups_per_eval": n * LOOKUPS_PER_LEVEL,
+            "frac": alg_glookups / LDS_PEAK["glookups"],
             "traffic": profiled_traffic(a.log2n),
             "traffic_unit": "bytes/launch (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0, 1 GiB copy calibration)",
             "kernel": "dcf_eval_queue_kernel<uint32_t, 8, true>",
             "kernel_ms": kernel_ms,
+            "lookups_per_eval": n * LOOKUPS_PER_LEVEL,
+            "peak_source": LDS_PEAK["source"],
+            "peak_words_per_clk_per_cu": live["words_per_clk_per_cu"],
+            "peak_recorded": recorded_lds_peak(),
+            "issued_lookups_per_eval": lds_per_eval,
+            "issued_frac": (lds_g / LDS_PEAK["glookups"]) if lds_g else None,
+            "valu_issued_per_eval": valu_per_eval,
+            "valu_issued_frac": (valu_t / VALU_PEAK_TOPS) if valu_t else None,
             "hbm_achieved_gbs": hbm_gbs,
             "hbm_frac": hbm_gbs / HBM_PEAK_GBS,
-            "measured_lookups_per_eval": lds_per_eval,
-            "measured_lds_frac": (lds_g / LDS_PEAK_GLOOKUPS) if lds_g else None,
-            "valu_model_ops_per_eval": n * OPS_PER_LEVEL,
-            "valu_model_tops": achieved_tops,
-            "valu_model_frac": achieved_tops / VALU_PEAK_TOPS,
-            "note": "bound: the LDS table-lookup pipe, which rocprof shows binding (VALU is idle part of the "
-                    "time). achieved: algorithmic lookups (per level one key schedule, 40, and two blocks, "
-                    "320). The kernel performs fewer, measured_lookups_per_eval (round 1 shared between the "
-                    "two blocks, unused last-round columns dropped); measured_lds_frac is the LDS pipe's "
-                    "actual load. valu_model_*: the 760-op/level VALU model, which also counts the work the "
-                    "kernel shares, so its fraction can pass 1",
+            "note": "bound: the LDS pipe that feeds the T-table AES. peak: the highest rate at which the LDS "
+                    "probe (probe/lds_probe.hip) gets 4-byte reads through on this box, in the same run. "
+                    "achieved: algorithmic lookups, per level one key schedule (40) and two blocks (320). "
+                    "The kernel issues fewer (issued_lookups_per_eval, rocprofv3 SQ_INSTS_LDS): the second "
+                    "block of a level shares round-1 and round-2 lookups with the first and its value block "
+                    "needs one last-round column, so frac can pass 1; issued_frac is the LDS pipe's actual "
+                    "load against the probe's ceiling. valu_issued_*: rocprofv3 SQ_INSTS_VALU against the "
+                    "VALU lane peak",
         },
+        "lds_probe": probe,
     }
-    del key, out
-    torch.cuda.empty_cache()
     progress("dcf: %.3f ms/step" % res["ms_per_step"])
     if not a.no_dpf:
         res["dpf"] = dpf_leg(a.log2n)

@@ -3,6 +3,8 @@
 Every ``csrc/*.hip`` and ``csrc/*.cpp`` is compiled with hipcc for
 ``--offload-arch=gfx950`` in parallel, then linked into one shared library
 exporting the ``FastFss_*`` C ABI declared under ``include/FastFss``.
+``probe/lds_probe.hip`` becomes ``libfss_lds_probe.so``, the LDS read-rate
+probe the benchmark measures its roofline ceiling with (not part of the C ABI).
 
 Usage: ``python -m fastfunctionsecretsharing_amd.build [-j N]``.
 """
@@ -20,6 +22,8 @@ PKG = Path(__file__).resolve().parent
 REPO = PKG.parent
 CSRC = PKG / "csrc"
 OUT = PKG / "libfastfss_hip.so"
+PROBE_SRC = PKG / "probe" / "lds_probe.hip"
+PROBE_OUT = PKG / "libfss_lds_probe.so"
 BUILD = PKG / "_build"
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
@@ -80,8 +84,16 @@ def build(jobs: int | None = None, verbose: bool = True, incremental: bool = Fal
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, OUT)
+    if not (incremental and PROBE_OUT.exists() and PROBE_OUT.stat().st_mtime >= PROBE_SRC.stat().st_mtime):
+        ptmp = PROBE_OUT.with_suffix(".so.tmp")
+        cmd = [HIPCC, *CFLAGS, "-shared", str(PROBE_SRC), "-o", str(ptmp)]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            msg = "\n".join(l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l)
+            raise RuntimeError(f"hipcc failed on {PROBE_SRC.name}:\n{msg}")
+        os.replace(ptmp, PROBE_OUT)
     if verbose:
-        print(f"built {OUT.relative_to(REPO)} from {len(srcs)} sources")
+        print(f"built {OUT.relative_to(REPO)} from {len(srcs)} sources and {PROBE_OUT.relative_to(REPO)}")
     return OUT
 
 
