@@ -67,6 +67,14 @@ SIGNATURES = {
     "FastFss_dcfSplineGetKeyDataSize": ("pssssss", "int"),
     "FastFss_dcfSplineGetZippedKeyDataSize": ("pssssss", "int"),
     "FastFss_otttGetKeyDataSize": ("pss", "int"),
+    # the host-buffer API (include/FastFss/cpu/*.h): the device entry points' arguments without the stream
+    "FastFss_cpu_setNumThreads": ("i", "int"),
+    "FastFss_cpu_getNumThreads": ("", "int"),
+    "FastFss_cpu_prngInit": ("", "ptr"),
+    "FastFss_cpu_prngRelease": ("p", None),
+    "FastFss_cpu_prngSetCurrentSeed": ("ppp", "int"),
+    "FastFss_cpu_prngGetCurrentSeed": ("ppp", "int"),
+    "FastFss_cpu_prngGen": ("ppsss", "int"),
     # MI355X-only entry points (include/FastFss/hip/*.h)
     "FastFss_hip_batchPlanCreate": ("pps", "int"),
     "FastFss_hip_batchPlanLaunch": ("pp", "int"),
@@ -157,6 +165,10 @@ GROTTO_FULL_LT, GROTTO_FULL_LE, GROTTO_FULL_EQ = 0, 1, 2
 BATCH_DCF_EVAL = 0
 BATCH_DPF_EVAL = 1
 
+for _name in [n for n in SIGNATURES if n.startswith("FastFss_cuda_") and "prng" not in n and "Grid" not in n]:
+    _kinds, _res = SIGNATURES[_name]
+    SIGNATURES[_name.replace("FastFss_cuda_", "FastFss_cpu_")] = (_kinds[:-1], _res)
+
 _KIND = {"p": ctypes.c_void_p, "s": ctypes.c_size_t, "i": ctypes.c_int, "b": ctypes.c_bool, "u": ctypes.c_uint32,
          "q": ctypes.c_uint64}
 _RES = {"int": ctypes.c_int, "ptr": ctypes.c_void_p, "size": ctypes.c_size_t, "uint": ctypes.c_uint32, None: None}
@@ -174,6 +186,13 @@ def load(path: Path | str | None = None):
     if _lib is not None and path is None:
         return _lib
     p = Path(path) if path else LIB_PATH
+    # torch bundles its own HIP runtime under the same sonames as the system's; if this library loaded
+    # first, its libamdhip64.so.7 would be the one a later `import torch` finds, and torch would then
+    # report no GPU.  Importing torch first (when installed) makes both use torch's runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not p.exists():
         raise LibraryMissing(
             f"{p} is not built; run `python -m fastfunctionsecretsharing_amd.build` "

@@ -1,0 +1,35 @@
+/* FastFss_cpu_* dpf entry points: the reference's host-buffer API, served on MI355X (gfx950).
+ * Pointers are host pointers.  Each call copies its input buffers to device memory on the calling
+ * thread's default HIP stream, runs the FastFss_cuda_* entry point of the same name (same validation,
+ * same error codes, same results), copies the outputs back and returns when they are in place.
+ * Without a usable GPU the calls return FAST_FSS_RUNTIME_ERROR. */
+
+#ifndef FAST_FSS_CPU_DPF_H
+#define FAST_FSS_CPU_DPF_H
+
+#include <FastFss/api.h>
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Replaces include/FastFss/cpu/dpf.h:12-26. */
+FAST_FSS_API int FastFss_cpu_dpfKeyGen(void *key, size_t keyDataSize, const void *alpha, size_t alphaDataSize, const void *beta, size_t betaDataSize, const void *seed0, size_t seedDataSize0, const void *seed1, size_t seedDataSize1, size_t bitWidthIn, size_t bitWidthOut, size_t groupSize, size_t elementSize, size_t elementNum);
+
+/* Replaces include/FastFss/cpu/dpf.h:28-43. */
+FAST_FSS_API int FastFss_cpu_dpfEval(void *sharedOut, size_t sharedOutDataSize, const void *maskedX, size_t maskedXDataSize, const void *key, size_t keyDataSize, const void *seed, size_t seedDataSize, int partyId, size_t bitWidthIn, size_t bitWidthOut, size_t groupSize, size_t elementSize, size_t elementNum, void *cache, size_t cacheDataSize);
+
+/* Replaces include/FastFss/cpu/dpf.h:45-60. */
+FAST_FSS_API int FastFss_cpu_dpfEvalAll(void *sharedOut, size_t sharedOutDataSize, const void *maskedX, size_t maskedXDataSize, const void *key, size_t keyDataSize, const void *seed, size_t seedDataSize, int partyId, size_t bitWidthIn, size_t bitWidthOut, size_t groupSize, size_t elementSize, size_t elementNum, void *cache, size_t cacheDataSize);
+
+/* Replaces include/FastFss/cpu/dpf.h:62-79. */
+FAST_FSS_API int FastFss_cpu_dpfEvalMulti(void *sharedOut, size_t sharedOutDataSize, const void *maskedX, size_t maskedXDataSize, const void *key, size_t keyDataSize, const void *seed, size_t seedDataSize, int partyId, const void *point, size_t pointDataSize, size_t bitWidthIn, size_t bitWidthOut, size_t groupSize, size_t elementSize, size_t elementNum, void *cache, size_t cacheDataSize);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -231,6 +231,24 @@ class OracleBackend:
         return ocall(name, *args)
 
 
+class HostBackend(OracleBackend):
+    """Runs the reference argument lists on numpy (host) buffers through the device library's
+    host-buffer entry points (FastFss_cpu_*), which evaluate on the GPU."""
+
+    name = "host-api"
+
+    def __init__(self):
+        from fastfunctionsecretsharing_amd import _lib
+
+        self.lib = _lib.load()
+
+    def call(self, name, *args):
+        prefix = "FastFss_" if name.startswith(("dcfGet", "dpfGet", "grottoGet", "dcfMICGet", "dcfSplineGet",
+                                                  "otttGet")) else "FastFss_cpu_"
+        return getattr(self.lib, prefix + name)(*[ptr(a) if (isinstance(a, np.ndarray) or a is None) else a
+                                                  for a in args])
+
+
 def _size(be_call, name, *args):
     out = ctypes.c_size_t(0)
     rc = be_call(name, ctypes.addressof(out), *args)

@@ -1,11 +1,12 @@
 """pyFastFss host-side behaviour without a GPU: the helper functions of the
 reference package (lib/torch_extension/pyFastFss/__init__.py:5-70), size
 queries equal to the oracle's, the reference's argument checks and reshaping,
-and a loud error (not a CPU fallback) for tensors that are not on a GPU."""
+and the routing of host tensors to the host-buffer entry points."""
 import pytest
 import torch
 
 import fss_testlib as F
+from fastfunctionsecretsharing_amd import _lib
 from fastfunctionsecretsharing_amd import pyFastFss as P
 
 
@@ -64,23 +65,50 @@ def _cpu_dcf_args(N=6, n=16, g=1):
     return alpha, beta, s
 
 
-def test_cpu_tensors_fail_loudly_after_reshaping():
+class _RecordingLib:
+    """Stands in for the device library: records each entry point called and returns an error code,
+    so host tensors can be routed without touching a GPU."""
+
+    def __init__(self, real, rc=-1):
+        self.real, self.rc, self.calls = real, rc, []
+
+    def __getattr__(self, name):
+        if name.startswith(("FastFss_cpu_", "FastFss_cuda_")) and not name.endswith(("NumThreads", "GridDim")):
+            def fn(*args):
+                self.calls.append((name, len(args)))
+                return None if name.endswith("prngInit") else self.rc
+            return fn
+        return getattr(self.real, name)
+
+
+def test_cpu_tensors_route_to_host_buffer_api(monkeypatch):
+    """CPU tensors go to FastFss_cpu_<op> (no stream argument), after the output is sized, as the
+    reference binding dispatches on device (csrc/dcf/dcf.cpp:134-138); a failing code raises
+    '<entry point> failed : <code>'."""
+    rec = _RecordingLib(P._L())
+    monkeypatch.setattr(P, "_L", lambda: rec)
     alpha, beta, s = _cpu_dcf_args()
     key = torch.empty(0, dtype=torch.uint8)
-    with pytest.raises(RuntimeError, match="GPU only"):
+    with pytest.raises(RuntimeError, match="FastFss_cpu_dcfKeyGen failed : -1"):
         P.dcf_key_gen(key, alpha, beta, s, s, 16, 16, 1)
-    # the key is sized before the device is dispatched, as in the reference
+    # the key is sized before the entry point is called, as in the reference
     assert key.numel() == P.dcf_get_key_data_size(16, 16, 1, 2, 6)
     out = torch.empty(0, dtype=torch.int16)
-    with pytest.raises(RuntimeError, match="GPU only"):
+    with pytest.raises(RuntimeError, match="FastFss_cpu_dcfEval failed"):
         P.dcf_eval(out, alpha, key, s, 0, 16, 16, 2)
     assert tuple(out.shape) == (2, 3, 2)
     out = torch.empty(0, dtype=torch.int16)
-    with pytest.raises(RuntimeError, match="GPU only"):
+    with pytest.raises(RuntimeError, match="FastFss_cpu_dpfEvalAll failed"):
         P.dpf_eval_all(out, alpha, key, s, 0, 4, 16, 1)
     assert tuple(out.shape) == (2, 3, 16)
-    with pytest.raises(RuntimeError, match="GPU only"):
+    with pytest.raises(RuntimeError, match="FastFss_cpu_prngInit failed"):
         P.Prng()
+    # the host-buffer signatures: the device ones without the trailing stream pointer
+    cuda_args = {n: len(k) for n, (k, _) in _lib.SIGNATURES.items()}
+    for name, nargs in rec.calls:
+        assert name.startswith("FastFss_cpu_")
+        if not name.endswith("prngInit"):
+            assert nargs == cuda_args[name.replace("cpu", "cuda")] - 1, name
 
 
 def test_argument_checks():
@@ -96,3 +124,11 @@ def test_argument_checks():
         P.dcf_eval(torch.empty(0, dtype=torch.int32), alpha, key, s, 0, 16, 16, 1)  # out dtype differs
     with pytest.raises(RuntimeError):
         P.grotto_eval(torch.empty(0, dtype=torch.int16), alpha, key, s.to(torch.int8), False, 0, 16)
+
+
+def test_top_level_import_is_the_same_module():
+    """Callers of the reference keep `import pyFastFss` (lib/torch_extension/setup.py:28-96)."""
+    import pyFastFss
+
+    assert pyFastFss is P
+    assert pyFastFss.dcf_eval is P.dcf_eval and pyFastFss.Prng is P.Prng
