@@ -552,6 +552,42 @@ def piecewise_leg(reps: int = 3):
     return res
 
 
+def serving_capacity(templates, picks, N=1 << 20, reps=3):
+    """Device-time bound of a serving trace.  Each class (op, bitWidthIn) of the trace is timed
+    alone through its per-call entry point at N elements (one table-holding workgroup per CU,
+    the kernels' saturated rate).  Only one such workgroup fits a CU (128 KB of LDS), so
+    launches in flight together share the CUs rather than overlap on them, and the trace needs
+    at least sum_i N_i / rate(class_i) of device time.  Returns (evals/s bound, rates)."""
+    import numpy as np
+    import torch
+
+    from fastfunctionsecretsharing_amd import workloads as W
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    holder, sp = W.stream_ptr(stream)
+    rng = np.random.default_rng(99)
+    gen = torch.Generator(device=dev).manual_seed(99)
+    rates = {}
+    for op, n in sorted({(templates[p].op, templates[p].n) for p in picks}):
+        t = W.make_template(op, n, N, rng, gen, dev, sp)
+        out = torch.empty(t.out_elems(), dtype=W.torch_dtype(t.es), device=dev)
+        cache = W.template_spline_cache(t) if t.degree else None
+        W.eval_template_alone(t, out, sp, cache)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            W.eval_template_alone(t, out, sp, cache)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        rates[(op, n)] = N * reps / (e0.elapsed_time(e1) / 1e3)
+        del t, out, cache
+        torch.cuda.empty_cache()
+    device_s = sum(templates[p].N / rates[(templates[p].op, templates[p].n)] for p in picks)
+    del holder
+    return sum(templates[p].N for p in picks) / device_s, rates
+
+
 def serving_leg(qps: float, requests: int, sweep_qps=(200, 800, "burst")):
     """Poisson arrivals at `qps`; each request a DCF, MIC (16 intervals) or
     spline (degree 2, 4 intervals) evaluation of 2^10..2^20 elements
@@ -573,7 +609,7 @@ def serving_leg(qps: float, requests: int, sweep_qps=(200, 800, "burst")):
     res = {"workload": "Poisson %.0f req/s, %d requests; DCF/MIC/spline 1:1:1, 2^10..2^20 elements log-uniform, "
                        "bitWidthIn in {16,32,64}; keys resident" % (qps, requests)}
     elems = sum(templates[p].N for p in picks)
-    forms = (("batched", (32 << 30, 16, 8, 1)), ("plain", (32 << 30, 1, 1, 0)))
+    forms = (("batched", (32 << 30, 16, 4, 1)), ("plain", (32 << 30, 1, 1, 0)))
 
     def run(arr, detail):
         out, row = {}, {}
@@ -586,6 +622,7 @@ def serving_leg(qps: float, requests: int, sweep_qps=(200, 800, "burst")):
             if detail:
                 row[name].update({
                     "mean_ms": float(lat_ms.mean()), "wall_s": wall, "max_inflight": int(st.maxInflightSeen),
+                    "launches": int(st.launches), "fused_requests": int(st.fusedRequests),
                     "config": {"hbm_budget_bytes": cfg[0], "max_inflight": cfg[1], "streams": cfg[2],
                                "launch": "one workgroup per 1024 elements" if cfg[3]
                                else "the per-call entry points' launch shape"}})
@@ -603,6 +640,18 @@ def serving_leg(qps: float, requests: int, sweep_qps=(200, 800, "burst")):
         row["qps"] = q
         sweep.append(row)
     res["qps_sweep"] = sweep
+    bound, rates = serving_capacity(templates, picks)
+    from fastfunctionsecretsharing_amd import _lib
+
+    names = {_lib.SERVE_DCF: "dcf", _lib.SERVE_MIC: "mic", _lib.SERVE_SPLINE: "spline", _lib.SERVE_DPF: "dpf"}
+    burst = sweep[-1]
+    res["device_bound"] = {
+        "evals_per_s": bound,
+        "class_rates_evals_per_s": {"%s_n%d" % (names[op], n): r for (op, n), r in sorted(rates.items())},
+        "burst_frac_batched": burst["batched"]["evals_per_s"] / bound,
+        "burst_frac_plain": burst["plain"]["evals_per_s"] / bound,
+        "note": "each class timed alone at 2^20 elements through its per-call entry point; the trace needs at "
+                "least sum N_i / rate(class_i) of device time, since one table-holding workgroup fills a CU"}
     return res
 
 

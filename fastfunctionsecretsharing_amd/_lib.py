@@ -135,6 +135,8 @@ class ServeStats(ctypes.Structure):
         ("maxInflightSeen", ctypes.c_uint64),
         ("elementsDone", ctypes.c_uint64),
         ("busySeconds", ctypes.c_double),
+        ("launches", ctypes.c_uint64),
+        ("fusedRequests", ctypes.c_uint64),
     ]
 
 

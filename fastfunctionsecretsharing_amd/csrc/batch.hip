@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cstring>
 #include <mutex>
 #include <numeric>
 #include <vector>
@@ -25,6 +26,7 @@
 #include "FastFss/hip/batch.h"
 #include "dpf_device.h"
 #include "fss_host.h"
+#include "fss_launch.h"
 
 namespace fss {
 
@@ -340,6 +342,34 @@ static int build_table(const FastFssBatchGroup *groups, size_t groupNum, std::ve
     }
     tilesOut = (uint32_t)tiles;
     return FAST_FSS_SUCCESS;
+}
+
+size_t batch_table_entry_bytes()
+{
+    return sizeof(BatchGroupDev);
+}
+
+int batch_enqueue_owned(const FastFssBatchGroup *groups, size_t groupNum, void *hostTab, void *devTab,
+                        uint32_t *devCounter, size_t cap, hipStream_t st)
+{
+    std::vector<BatchGroupDev> host;
+    uint32_t                   tiles = 0;
+    int                        rc    = build_table(groups, groupNum, host, tiles);
+    if (rc) return rc;
+    if (host.size() > cap || !hostTab || !devTab || !devCounter) return FAST_FSS_RUNTIME_ERROR;
+    if (tiles == 0) return FAST_FSS_SUCCESS;
+    const size_t bytes = sizeof(BatchGroupDev) * host.size();
+    std::memcpy(hostTab, host.data(), bytes);
+    // pinned source: the copy is asynchronous; the caller reuses hostTab only
+    // after work recorded behind this launch has completed
+    if (hipMemcpyAsync(devTab, hostTab, bytes, hipMemcpyHostToDevice, st) != hipSuccess)
+        return FAST_FSS_RUNTIME_ERROR;
+    BatchPlan p;
+    p.groups  = (BatchGroupDev *)devTab;
+    p.counter = devCounter;
+    p.G       = (int)host.size();
+    p.tiles   = tiles;
+    return plan_enqueue(&p, st);
 }
 
 } // namespace fss

@@ -12,6 +12,8 @@
 
 #include "fss_host.h"
 
+struct FastFssBatchGroup;
+
 namespace fss {
 
 // Zeroed 32-bit work-queue counter in stream-ordered memory on `st`
@@ -39,5 +41,17 @@ int launch_spline_eval_rt(size_t es, int grid, hipStream_t st, void *out, const 
                           const void *e, const void *beta, const void *seed, int party, const void *left,
                           const void *right, size_t I, int degree, int n, int m, size_t N, uint32_t *counter,
                           int block = kBlock);
+
+// Bytes of one entry of the device group table of a fused batch launch.
+size_t batch_table_entry_bytes();
+
+// One persistent batch launch over `groupNum` groups (validated like
+// FastFss_hip_batchEval) with tables the caller owns: the group table is
+// built in `hostTab` (pinned host memory, `cap` entries), copied to `devTab`
+// on `st`, and `devCounter` (4 bytes) is reset on `st` before the launch.
+// The caller may reuse the tables once work recorded after the launch on
+// `st` has completed.  Allocates nothing and does not synchronize.
+int batch_enqueue_owned(const FastFssBatchGroup *groups, size_t groupNum, void *hostTab, void *devTab,
+                        uint32_t *devCounter, size_t cap, hipStream_t st);
 
 } // namespace fss

@@ -1,10 +1,17 @@
 // Continuous-batching request server (see include/FastFss/hip/serve.h).
 //
-// One scheduler thread owns all device work: it admits queued requests in
-// arrival order under the in-flight and HBM-budget limits, launches each on
-// the least recently used of its streams, records a completion event and
-// retires requests whose events have fired.  Client threads only touch the
-// queue and the request records under the mutex.
+// One scheduler thread owns all device work.  Each pass it
+//   1. queries the completion events of the launches in flight (no lock),
+//   2. under the mutex: retires the requests of finished launches and admits
+//      queued requests in arrival order under the in-flight and HBM-budget
+//      limits,
+//   3. without the mutex: launches what it admitted.  DCF and DPF requests
+//      admitted in the same pass share one persistent batch launch (the
+//      ragged-batch kernel); MIC and spline requests, and a lone DCF or DPF
+//      request, get a launch each.  Each launch goes to the stream with the
+//      fewest launches in flight and is followed by a completion event.
+// Client threads only touch the queue and the request records, under the
+// mutex, so a submit never waits behind a kernel launch.
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -14,6 +21,7 @@
 #include <unordered_map>
 #include <vector>
 
+#include "FastFss/hip/batch.h"
 #include "FastFss/hip/serve.h"
 #include "fss_host.h"
 #include "fss_launch.h"
@@ -29,24 +37,52 @@ struct ServeItem
     size_t              bytes;
     Clock::time_point   tSubmit, tDone;
     int                 status = 1; // 1 queued, 2 running, 0 done, <0 failed
-    hipEvent_t          ev     = nullptr;
+};
+
+// One launch in flight: the requests it serves and its completion event.
+struct Launch
+{
+    std::vector<ServeItem *> items;
+    hipEvent_t               ev     = nullptr;
+    int                      stream = 0;
+    int                      slot   = -1; // fused-table slot, -1 for a single request
+};
+
+// Tables of one fused batch launch: the group table in pinned host memory,
+// its device copy and the work-queue counter.
+struct FusedSlot
+{
+    void     *host    = nullptr;
+    void     *dev     = nullptr;
+    uint32_t *counter = nullptr;
+    bool      busy    = false;
 };
 
 struct Server
 {
-    std::mutex                                            mu;
-    std::condition_variable                               cvWork, cvDone;
-    std::deque<ServeItem *>                               queue;
-    std::vector<ServeItem *>                              running;
+    // shared with client threads, under mu
+    std::mutex                                               mu;
+    std::condition_variable                                  cvWork, cvDone;
+    std::deque<ServeItem *>                                  queue;
     std::unordered_map<uint64_t, std::unique_ptr<ServeItem>> items;
-    std::vector<hipStream_t>                              streams;
-    size_t                                                budget, inflightBytes = 0;
-    int                                                   maxInflight, launchMode, device = 0, nextStream = 0;
-    uint64_t                                              nextId = 1;
-    bool                                                  stop   = false;
-    FastFssServeStats                                     stats{};
-    Clock::time_point                                     busySince;
-    std::thread                                           th;
+    size_t                                                   inflightBytes = 0;
+    int                                                      inflightItems = 0;
+    uint64_t                                                 nextId        = 1;
+    bool                                                     stop          = false;
+    FastFssServeStats                                        stats{};
+    Clock::time_point                                        busySince;
+    // set at creation
+    size_t budget;
+    int    maxInflight, launchMode, device = 0;
+    // owned by the scheduler thread
+    std::vector<hipStream_t> streams;
+    std::vector<int>         streamLoad;
+    int                      nextStream = 0;
+    std::vector<Launch>      running;
+    std::vector<FusedSlot>   slots;
+    std::vector<hipEvent_t>  eventPool;
+    uint64_t                 newLaunches = 0, newFused = 0; // added to stats at the next locked step
+    std::thread              th;
 };
 
 static size_t request_bytes(const FastFssServeRequest &r)
@@ -126,78 +162,214 @@ static int launch(Server *s, ServeItem *it, hipStream_t st)
     }
 }
 
+static int pick_stream(Server *s)
+{
+    const int k    = (int)s->streams.size();
+    int       best = s->nextStream;
+    for (int j = 1; j < k; j++)
+    {
+        const int c = (s->nextStream + j) % k;
+        if (s->streamLoad[c] < s->streamLoad[best]) best = c;
+    }
+    s->nextStream = (best + 1) % k;
+    return best;
+}
+
+static hipEvent_t take_event(Server *s)
+{
+    if (!s->eventPool.empty())
+    {
+        hipEvent_t e = s->eventPool.back();
+        s->eventPool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    return hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess ? e : nullptr;
+}
+
+// Enqueue launch L (its items and slot already set) on a stream and record
+// its completion event.  Returns a FastFssError.
+static int start_launch(Server *s, Launch &L)
+{
+    L.stream       = pick_stream(s);
+    hipStream_t st = s->streams[L.stream];
+    int         rc = FAST_FSS_SUCCESS;
+    if (L.slot >= 0)
+    {
+        std::vector<FastFssBatchGroup> gs(L.items.size());
+        for (size_t k = 0; k < L.items.size(); k++)
+        {
+            const FastFssServeRequest &r = L.items[k]->req;
+            FastFssBatchGroup         &g = gs[k];
+            g.op          = r.op == FAST_FSS_SERVE_DCF_EVAL ? FAST_FSS_BATCH_DCF_EVAL : FAST_FSS_BATCH_DPF_EVAL;
+            g.partyId     = r.partyId;
+            g.bitWidthIn  = r.bitWidthIn;
+            g.bitWidthOut = r.bitWidthOut;
+            g.elementSize = r.elementSize;
+            g.elementNum  = r.elementNum;
+            g.sharedOut   = r.sharedOut;
+            g.maskedX     = r.maskedX;
+            g.key         = r.key;
+            g.seed        = r.seed;
+            g.keyDataSize = r.keyDataSize;
+            g.tileElems   = 0;
+        }
+        const FusedSlot &f = s->slots[L.slot];
+        rc = batch_enqueue_owned(gs.data(), gs.size(), f.host, f.dev, f.counter, (size_t)s->maxInflight, st);
+    }
+    else
+        rc = launch(s, L.items[0], st);
+    if (rc != FAST_FSS_SUCCESS) return rc;
+    L.ev = take_event(s);
+    if (!L.ev) return FAST_FSS_RUNTIME_ERROR;
+    if (hipEventRecord(L.ev, st) != hipSuccess)
+    {
+        s->eventPool.push_back(L.ev);
+        L.ev = nullptr;
+        return FAST_FSS_RUNTIME_ERROR;
+    }
+    return FAST_FSS_SUCCESS;
+}
+
+// Group the requests admitted in one pass into launches and start them;
+// requests whose launch failed go to `failed` with their error code.
+static void launch_admitted(Server *s, const std::vector<ServeItem *> &admitted,
+                            std::vector<std::pair<ServeItem *, int>> &failed)
+{
+    std::vector<ServeItem *> fuse;
+    std::vector<Launch>      launches;
+    for (ServeItem *it : admitted)
+    {
+        const int op = it->req.op;
+        if ((op == FAST_FSS_SERVE_DCF_EVAL || op == FAST_FSS_SERVE_DPF_EVAL) && it->req.elementNum > 0)
+            fuse.push_back(it);
+        else
+        {
+            Launch L;
+            L.items.push_back(it);
+            launches.push_back(std::move(L));
+        }
+    }
+    int slot = -1;
+    if (fuse.size() >= 2)
+        for (int k = 0; k < (int)s->slots.size() && slot < 0; k++)
+            if (!s->slots[k].busy) slot = k;
+    if (slot >= 0)
+    {
+        Launch L;
+        L.items = fuse;
+        L.slot  = slot;
+        launches.insert(launches.begin(), std::move(L));
+    }
+    else
+        for (ServeItem *it : fuse)
+        {
+            Launch L;
+            L.items.push_back(it);
+            launches.insert(launches.begin(), std::move(L));
+        }
+    for (Launch &L : launches)
+    {
+        const int rc = start_launch(s, L);
+        if (rc != FAST_FSS_SUCCESS)
+        {
+            for (ServeItem *it : L.items) failed.emplace_back(it, rc);
+            continue;
+        }
+        if (L.slot >= 0)
+        {
+            s->slots[L.slot].busy = true;
+            s->newFused += L.items.size();
+        }
+        s->newLaunches++;
+        s->streamLoad[L.stream]++;
+        s->running.push_back(std::move(L));
+    }
+}
+
+// Mark `it` finished with `status` (under the mutex).
+static void retire(Server *s, ServeItem *it, int status, Clock::time_point now)
+{
+    it->tDone  = now;
+    it->status = status;
+    s->inflightBytes -= it->bytes;
+    s->inflightItems--;
+    if (status == 0)
+    {
+        s->stats.completed++;
+        s->stats.elementsDone += it->req.elementNum;
+    }
+    else
+        s->stats.failed++;
+    if (s->inflightItems == 0) s->stats.busySeconds += std::chrono::duration<double>(now - s->busySince).count();
+}
+
 static void scheduler(Server *s)
 {
     (void)hipSetDevice(s->device);
-    std::unique_lock<std::mutex> lock(s->mu);
+    std::vector<ServeItem *>                 admitted;
+    std::vector<std::pair<ServeItem *, int>> finished, failed;
     while (true)
     {
-        // retire finished requests
+        // 1. completions (scheduler-owned state, no lock)
+        finished.clear();
         for (size_t i = 0; i < s->running.size();)
         {
-            ServeItem  *it = s->running[i];
-            hipError_t q  = hipEventQuery(it->ev);
+            Launch          &L = s->running[i];
+            const hipError_t q = hipEventQuery(L.ev);
             if (q == hipErrorNotReady)
             {
                 i++;
                 continue;
             }
-            it->tDone  = Clock::now();
-            it->status = (q == hipSuccess) ? 0 : FAST_FSS_RUNTIME_ERROR;
-            (void)hipEventDestroy(it->ev);
-            it->ev = nullptr;
-            s->inflightBytes -= it->bytes;
-            if (it->status == 0)
-            {
-                s->stats.completed++;
-                s->stats.elementsDone += it->req.elementNum;
-            }
-            else
-                s->stats.failed++;
-            s->running[i] = s->running.back();
+            const int status = q == hipSuccess ? 0 : FAST_FSS_RUNTIME_ERROR;
+            for (ServeItem *it : L.items) finished.emplace_back(it, status);
+            s->eventPool.push_back(L.ev);
+            s->streamLoad[L.stream]--;
+            if (L.slot >= 0) s->slots[L.slot].busy = false;
+            s->running[i] = std::move(s->running.back());
             s->running.pop_back();
-            if (s->running.empty())
-                s->stats.busySeconds += std::chrono::duration<double>(it->tDone - s->busySince).count();
-            s->cvDone.notify_all();
         }
-        // admit in arrival order
-        while (!s->queue.empty() && (int)s->running.size() < s->maxInflight)
+        // 2. retire and admit
+        admitted.clear();
         {
-            ServeItem *it = s->queue.front();
-            if (!s->running.empty() && s->inflightBytes + it->bytes > s->budget) break;
-            s->queue.pop_front();
-            hipStream_t st = s->streams[s->nextStream];
-            s->nextStream  = (s->nextStream + 1) % (int)s->streams.size();
-            int rc         = launch(s, it, st);
-            if (rc == FAST_FSS_SUCCESS && hipEventCreateWithFlags(&it->ev, hipEventDisableTiming) != hipSuccess)
-                rc = FAST_FSS_RUNTIME_ERROR;
-            if (rc == FAST_FSS_SUCCESS && hipEventRecord(it->ev, st) != hipSuccess) rc = FAST_FSS_RUNTIME_ERROR;
-            if (rc != FAST_FSS_SUCCESS)
+            std::unique_lock<std::mutex> lock(s->mu);
+            s->stats.launches += s->newLaunches;
+            s->stats.fusedRequests += s->newFused;
+            s->newLaunches = s->newFused = 0;
+            if (!finished.empty() || !failed.empty())
             {
-                if (it->ev) (void)hipEventDestroy(it->ev);
-                it->ev     = nullptr;
-                it->status = rc;
-                it->tDone  = Clock::now();
-                s->stats.failed++;
+                const auto now = Clock::now();
+                for (auto &f : finished) retire(s, f.first, f.second, now);
+                for (auto &f : failed) retire(s, f.first, f.second, now);
+                failed.clear();
                 s->cvDone.notify_all();
+            }
+            while (!s->queue.empty() && s->inflightItems < s->maxInflight)
+            {
+                ServeItem *it = s->queue.front();
+                if (s->inflightItems > 0 && s->inflightBytes + it->bytes > s->budget) break;
+                s->queue.pop_front();
+                if (s->inflightItems == 0) s->busySince = Clock::now();
+                it->status = 2;
+                s->inflightItems++;
+                s->inflightBytes += it->bytes;
+                admitted.push_back(it);
+            }
+            if ((uint64_t)s->inflightItems > s->stats.maxInflightSeen) s->stats.maxInflightSeen = s->inflightItems;
+            if (s->inflightItems == 0)
+            {
+                if (s->stop && s->queue.empty()) break;
+                s->cvWork.wait(lock, [&] { return s->stop || !s->queue.empty(); });
                 continue;
             }
-            if (s->running.empty()) s->busySince = Clock::now();
-            it->status = 2;
-            s->running.push_back(it);
-            s->inflightBytes += it->bytes;
-            if (s->running.size() > s->stats.maxInflightSeen) s->stats.maxInflightSeen = s->running.size();
         }
-        if (s->stop && s->queue.empty() && s->running.empty()) break;
-        if (s->running.empty())
-            s->cvWork.wait(lock, [&] { return s->stop || !s->queue.empty(); });
+        // 3. launches (no lock)
+        if (!admitted.empty())
+            launch_admitted(s, admitted, failed);
         else
-        {
             // poll completions at a fine grain while work is in flight
-            lock.unlock();
             std::this_thread::sleep_for(std::chrono::microseconds(20));
-            lock.lock();
-        }
     }
 }
 
@@ -205,30 +377,51 @@ static void scheduler(Server *s)
 
 using namespace fss;
 
+static void server_free(Server *s)
+{
+    for (auto st : s->streams) (void)hipStreamDestroy(st);
+    for (auto e : s->eventPool) (void)hipEventDestroy(e);
+    for (auto &f : s->slots)
+    {
+        if (f.host) (void)hipHostFree(f.host);
+        if (f.dev) (void)hipFree(f.dev);
+        if (f.counter) (void)hipFree(f.counter);
+    }
+    delete s;
+}
+
 extern "C" FAST_FSS_API int FastFss_hip_serverCreate(void **server, size_t hbmBudgetBytes, int maxInflight,
                                                      int numStreams, int launchMode)
 {
-    if (!server || maxInflight < 1 || numStreams < 1 || numStreams > 64 || (launchMode != 0 && launchMode != 1))
+    if (!server || maxInflight < 1 || maxInflight > 4096 || numStreams < 1 || numStreams > 64 ||
+        (launchMode != 0 && launchMode != 1))
         return FAST_FSS_RUNTIME_ERROR;
     Server *s      = new Server();
     s->budget      = hbmBudgetBytes;
     s->maxInflight = maxInflight;
     s->launchMode  = launchMode;
-    if (hipGetDevice(&s->device) != hipSuccess)
-    {
-        delete s;
-        return FAST_FSS_RUNTIME_ERROR;
-    }
-    for (int i = 0; i < numStreams; i++)
+    bool ok        = hipGetDevice(&s->device) == hipSuccess;
+    for (int i = 0; ok && i < numStreams; i++)
     {
         hipStream_t st;
-        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess)
-        {
-            for (auto x : s->streams) (void)hipStreamDestroy(x);
-            delete s;
-            return FAST_FSS_RUNTIME_ERROR;
-        }
-        s->streams.push_back(st);
+        ok = hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess;
+        if (ok) s->streams.push_back(st);
+    }
+    s->streamLoad.assign(s->streams.size(), 0);
+    // a fused launch serves two or more requests, so maxInflight / 2 of them
+    // can be in flight at once
+    if (maxInflight >= 2) s->slots.resize((size_t)maxInflight / 2);
+    const size_t tab = batch_table_entry_bytes() * (size_t)maxInflight;
+    for (auto &f : s->slots)
+    {
+        if (!ok) break;
+        ok = hipHostMalloc(&f.host, tab, hipHostMallocDefault) == hipSuccess &&
+             hipMalloc(&f.dev, tab) == hipSuccess && hipMalloc((void **)&f.counter, sizeof(uint32_t)) == hipSuccess;
+    }
+    if (!ok)
+    {
+        server_free(s);
+        return FAST_FSS_RUNTIME_ERROR;
     }
     s->th   = std::thread(scheduler, s);
     *server = s;
@@ -296,7 +489,7 @@ extern "C" FAST_FSS_API int FastFss_hip_serverDrain(void *server)
     Server *s = (Server *)server;
     if (!s) return FAST_FSS_RUNTIME_ERROR;
     std::unique_lock<std::mutex> lock(s->mu);
-    s->cvDone.wait(lock, [&] { return s->queue.empty() && s->running.empty(); });
+    s->cvDone.wait(lock, [&] { return s->queue.empty() && s->inflightItems == 0; });
     return s->stats.failed ? FAST_FSS_RUNTIME_ERROR : FAST_FSS_SUCCESS;
 }
 
@@ -319,6 +512,5 @@ extern "C" FAST_FSS_API void FastFss_hip_serverDestroy(void *server)
         s->cvWork.notify_all();
     }
     s->th.join();
-    for (auto st : s->streams) (void)hipStreamDestroy(st);
-    delete s;
+    server_free(s);
 }

@@ -283,8 +283,19 @@ def serve_request(t, out):
     return r
 
 
-def eval_template_alone(t, out, sp):
-    """The per-call public entry point for one request (the reference-shaped launch)."""
+def template_spline_cache(t):
+    """The cache buffer FastFss_cuda_dcfSplineEval takes for serving template `t`."""
+    import torch
+
+    cs = ctypes.c_size_t(0)
+    assert _lib.load().FastFss_dcfSplineGetCacheDataSize(ctypes.addressof(cs), t.degree, t.intervals, t.n, t.n, t.es,
+                                                         t.N) == 0
+    return torch.empty(cs.value, dtype=torch.uint8, device=t.key.device)
+
+
+def eval_template_alone(t, out, sp, cache=None):
+    """The per-call public entry point for one request (the reference-shaped launch).  A spline
+    call without `cache` allocates one and synchronizes before releasing it."""
     import torch
 
     lib = _lib.load()
@@ -301,17 +312,18 @@ def eval_template_alone(t, out, sp):
                                          sp)
     else:
         I, d = t.intervals, t.degree
-        cs = ctypes.c_size_t(0)
-        assert lib.FastFss_dcfSplineGetCacheDataSize(ctypes.addressof(cs), d, I, n, n, es, N) == 0
-        cache = torch.empty(cs.value, dtype=torch.uint8, device=out.device)
+        own = cache is None
+        if own:
+            cache = template_spline_cache(t)
         g = I * (d + 1)
         rc = lib.FastFss_cuda_dcfSplineEval(out.data_ptr(), es * N, t.masked.data_ptr(), es * N, t.key.data_ptr(),
                                             t.key_size, t.z.data_ptr(), es * N * g, t.beta.data_ptr(), es * N * g,
                                             t.seed0.data_ptr(), 16 * N, 0, t.left.data_ptr(), es * I,
                                             t.right.data_ptr(), es * I, I, d, n, n, es, N, cache.data_ptr(),
-                                            cs.value, sp)
-        torch.cuda.synchronize()
-        del cache
+                                            cache.numel(), sp)
+        if own:
+            torch.cuda.synchronize()
+            del cache
     if rc != 0:
         raise RuntimeError(f"eval failed: {rc}")
 

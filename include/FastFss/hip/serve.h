@@ -7,11 +7,22 @@
  *   - the number of requests in flight is below `maxInflight`, and
  *   - the bytes they touch (key + inputs + outputs) fit the HBM budget
  *     (a request larger than the budget is admitted alone),
- * launches each admitted request on one of `numStreams` HIP streams and
- * retires it when its completion event fires.  Work of concurrently admitted
- * requests overlaps on the device: with launchMode 1 every request uses one
- * workgroup per 1024 elements, so the hardware interleaves the workgroups of
- * all in-flight requests across the CUs.
+ * launches the admitted requests on `numStreams` HIP streams (each launch on
+ * the stream with the fewest launches in flight) and retires them when their
+ * completion events fire.  The scheduler launches without holding the lock
+ * that Submit, Query and Wait take.
+ *
+ * DCF and DPF requests admitted together (two or more in one scheduling pass)
+ * share one persistent launch of the ragged-batch kernel
+ * (include/FastFss/hip/batch.h), which pulls tiles of all of them from one work
+ * queue.  MIC and spline requests, and a DCF or DPF request admitted alone, get
+ * a launch each.  Work of concurrently admitted launches overlaps on the
+ * device: with launchMode 1 every single-request launch uses one workgroup per
+ * 1024 elements, so the hardware interleaves the workgroups of all launches in
+ * flight across the CUs.
+ *
+ * The runtime gives each process GPU_MAX_HW_QUEUES hardware queues (4 unless
+ * set); streams beyond that share queues and add no concurrency.
  *
  * Each request's output is bit-identical to the per-call entry point with
  * the same arguments (FastFss_cuda_dcfEval, FastFss_cuda_dpfEval,
@@ -65,7 +76,9 @@ typedef struct FastFssServeStats
     uint64_t failed;
     uint64_t maxInflightSeen;
     uint64_t elementsDone;
-    double   busySeconds; /* wall time with at least one request in flight */
+    double   busySeconds;   /* wall time with at least one request in flight */
+    uint64_t launches;      /* kernel launches made (a fused launch counts once) */
+    uint64_t fusedRequests; /* requests served by fused DCF/DPF batch launches */
 } FastFssServeStats;
 
 /* launchMode: 0 = the per-call entry points' shape (one 1024-lane workgroup

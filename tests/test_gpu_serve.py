@@ -63,6 +63,22 @@ def test_mix_with_dpf_equals_alone(env):
         assert torch.equal(a, b), (i, templates[picks[i]].op)
 
 
+def test_fused_dcf_dpf_burst(env):
+    """A burst of DCF and DPF requests of mixed widths: requests admitted together share fused
+    batch launches (fewer launches than requests), and each equals its alone result."""
+    torch, _lib, W, _ = env
+    templates = W.serving_templates(count=10, seed=33, lo=5, hi=14, widths=(8, 16, 32, 64),
+                                    ops=(_lib.SERVE_DCF, _lib.SERVE_DPF))
+    rng = np.random.default_rng(8)
+    picks = [int(p) for p in rng.integers(0, len(templates), size=48)]
+    outs, lats, wall, st = W.run_trace(templates, picks, np.zeros(len(picks)), 1 << 34, 16, 4, 1)
+    assert st.completed == len(picks) and st.failed == 0
+    assert st.fusedRequests >= 2 and st.launches < len(picks), (st.launches, st.fusedRequests)
+    ref = _alone(torch, W, templates, picks)
+    for i, (a, b) in enumerate(zip(outs, ref)):
+        assert torch.equal(a, b), (i, templates[picks[i]].op, templates[picks[i]].n, templates[picks[i]].N)
+
+
 def test_submit_validation(env):
     import ctypes
 
@@ -155,7 +171,7 @@ def test_serving_bench_trace(env):
     arrivals = np.cumsum(rng.exponential(1.0 / 400, size=400))
     sizes = [templates[p].N for p in picks]
     assert min(sizes) >= 1 << 10 and max(sizes) <= 1 << 20 and max(sizes) > 1 << 19
-    outs, lats, wall, st = W.run_trace(templates, picks, arrivals, 32 << 30, 16, 8, 1)
+    outs, lats, wall, st = W.run_trace(templates, picks, arrivals, 32 << 30, 16, 4, 1)
     assert st.completed == len(picks) and st.failed == 0
     print(f"served 400 requests in {wall:.2f} s, max in flight {st.maxInflightSeen}", flush=True)
     ref = _alone(torch, W, templates, picks)
