@@ -42,10 +42,46 @@ TRACE_PROFILE = REPO / "profiles" / "dcf_eval_queue_pre_n32_2p24_kernel_trace.md
 # box it runs on and takes the live figure as the roofline peak
 LDS_PEAK_PROFILE = REPO / "profiles" / "lds_peak.md"
 # probe layouts whose reads are 4-byte lookups (one or two per instruction)
-LDS_LOOKUP_LAYOUTS = (0, 1, 5, 7)
-# Algorithmic AES table lookups per DCF level (taken child only): one key
-# schedule (10 rounds x 4 S-box lookups) and two blocks (10 rounds x 16)
-LOOKUPS_PER_LEVEL = 10 * 4 + 2 * 10 * 16
+LDS_LOOKUP_LAYOUTS = (0, 1, 5, 7, 8, 9)
+# Algorithmic AES table lookups.  A key schedule is 10 rounds x 4 S-box
+# lookups and a block 10 rounds x 16.  Blocks under one key whose plaintexts
+# differ only in low bytes need fewer, whatever implements them: after the
+# whitening they differ only in byte 0 of some words, so those words' lookups
+# are the only ones of round 1 they do not share, and round 2 shares every
+# lookup on a word that round 1 left equal.
+#   - a second PRG counter block (plaintexts (4x, 4x+1), (4x+2, 4x+3), which
+#     differ in byte 0 of words 0 and 2): 14 lookups of round 1 and the 8 of
+#     round 2 on words 1 and 3 are shared, so it costs 160 - 14 - 8 = 138;
+#   - each further value-conversion window (plaintexts (0, 0, c, 0), which
+#     differ in byte 0 of word 2): 15 of round 1 and the 12 of round 2 on
+#     words 0, 1 and 3, so it costs 160 - 15 - 12 = 133;
+#   - a block of which only the low k of 4 output columns are used needs
+#     only those columns of the last round: 4 (4 - k) lookups fewer.
+LK_SCHED, LK_BLOCK, LK_CTR2, LK_WIN2 = 40, 160, 160 - 14 - 8, 160 - 15 - 12
+
+
+def lk_last_cut(out_bytes: int) -> int:
+    """Last-round lookups not needed when only the low out_bytes bytes of a block are used."""
+    return 4 * (4 - min(4, (out_bytes + 3) // 4))
+
+
+def lk_dcf_level(out_bytes: int) -> int:
+    """One DCF eval level, groupSize 1: key schedule, the taken child's seed block and value block
+    (a second counter block, of which only the low out_bytes bytes are used)."""
+    return LK_SCHED + LK_BLOCK + LK_CTR2 - lk_last_cut(out_bytes)
+
+
+def lk_dpf_eval(n: int, out_bytes: int) -> int:
+    """One DPF eval, groupSize 1: n levels of key schedule + the taken child's block, then the value
+    conversion of the leaf seed (AES_s(0) when bitWidthOut > 16, the seed's own bytes otherwise)."""
+    conv = LK_SCHED + LK_BLOCK - lk_last_cut(out_bytes) if out_bytes > 2 else 0
+    return n * (LK_SCHED + LK_BLOCK) + conv
+
+
+# Algorithmic AES table lookups per level of the headline DCF eval (bitWidthOut = 32): 326
+LOOKUPS_PER_LEVEL = lk_dcf_level(4)
+# The survey's per-level figure, which charges both blocks in full: 360
+SURVEY_LOOKUPS_PER_LEVEL = LK_SCHED + 2 * LK_BLOCK
 
 
 def profiled_value(path: Path, key: str):
@@ -234,23 +270,18 @@ def cpu_sized_workload_gpu(reps: int = 20):
             "device_ms_both_parties": dev_ms, "reconstruction_ok": ok}
 
 
-# AES table lookups of one key schedule and of one block
-LDS_COST = (10 * 4, 10 * 16)
-
-
-def ragged_ops(groups, cost=LDS_COST):
-    """Algorithmic AES table lookups of one pass over a ragged batch: DCF n x (schedule + 2 blocks),
-    DPF n x (schedule + block) plus one more level for the value conversion when bitWidthOut > 16.
-    `cost` = lookups of (key schedule, block)."""
+def ragged_ops(groups):
+    """Algorithmic AES table lookups of one pass over a ragged batch (bitWidthOut = bitWidthIn):
+    lk_dcf_level per DCF level, lk_dpf_eval per DPF eval."""
     from fastfunctionsecretsharing_amd import _lib
 
-    ks, blk = cost
     ops = 0
     for g in groups:
+        ob = (g.n + 7) // 8
         if g.op == _lib.BATCH_DCF_EVAL:
-            ops += g.N * g.n * (ks + 2 * blk)
+            ops += g.N * g.n * lk_dcf_level(ob)
         else:
-            ops += g.N * (g.n + (1 if g.n > 16 else 0)) * (ks + blk)
+            ops += g.N * lk_dpf_eval(g.n, ob)
     return ops
 
 
@@ -367,7 +398,7 @@ def dpf_leg(log2n: int, reps: int = 10):
     del y0, rec
     step()
     wall, dev_ms = _timed(torch.cuda.current_stream(), step, reps)
-    lookups = (n + 1) * sum(LDS_COST)  # n levels + the value conversion's AES (bitWidthOut > 16)
+    lookups = lk_dpf_eval(n, 4)
     del holder
     return {"workload": "plain DPF eval, bitWidthIn=32, bitWidthOut=32, groupSize=1, 2^%d elements, party 0" % log2n,
             "parity_ok": ok, "evals_per_s": N / wall, "kernel_ms": dev_ms, "lookups_per_eval": lookups,
@@ -383,6 +414,8 @@ LDS_PROBE_LAYOUTS = {
     5: "ds_read2st64_b32 (two 4-byte reads per instruction), 256-B rows",
     6: "as 0 with 8 reads in flight per wave instead of 32",
     7: "ds_read_b32, 32 per asm block at immediate offsets from one address register, 256-B rows, 64 banks",
+    8: "as 0, software-pipelined: the next 16 reads issued before the previous 16 are consumed",
+    9: "as 8 with one address register: ds_read2st64_b32 (two 4-byte reads per instruction)",
 }
 
 
@@ -472,17 +505,16 @@ def ottt_leg(reps: int = 10):
     return res
 
 
-def spline_ops(w, cost=LDS_COST):
+def spline_ops(w):
     """Algorithmic AES table lookups of one spline element: the range walks of the
     interval sweep (one per interval end, plus one per interval start that does
-    not follow its predecessor), each n levels of (key schedule + 2 blocks) for
-    the seed and (key schedule + one block per 16-byte window of its slots) for
-    the value conversion, plus the final conversion."""
+    not follow its predecessor), each n levels of (key schedule + the taken
+    child's seed and value blocks) and (key schedule + one block per 16-byte
+    window of its slots) for the value conversion, plus the final conversion."""
     left = [int(v) for v in w.left.cpu().tolist()]
     right = [int(v) for v in w.right.cpu().tolist()]
     mask = (1 << w.n) - 1
     C, OB = w.d + 1, (w.m + 7) // 8
-    ks, blk = cost
     g = w.I * C
 
     def walk(j0, cnt):
@@ -490,8 +522,8 @@ def spline_ops(w, cost=LDS_COST):
             conv = 0
         else:
             nw = ((j0 + cnt) * OB - 1) // 16 - (j0 * OB) // 16 + 1
-            conv = ks + nw * blk
-        return w.n * (ks + 2 * blk + conv) + conv
+            conv = LK_SCHED + LK_BLOCK + (nw - 1) * LK_WIN2
+        return w.n * (LK_SCHED + LK_BLOCK + LK_CTR2 + conv) + conv
 
     ops = 0
     for q in range(w.I):
@@ -518,7 +550,7 @@ def piecewise_leg(reps: int = 3):
     W.mic_eval(w, 0, out, w.z, sp)
     wall, dev_ms = _timed(stream, lambda: W.mic_eval(w, 0, out, w.z, sp), reps)
     # contiguous intervals share endpoints: I + 1 DCF walks per element
-    lookups = (I + 1) * w.n * (LDS_COST[0] + 2 * LDS_COST[1])
+    lookups = (I + 1) * w.n * lk_dcf_level((w.m + 7) // 8)
     res["mic"] = {"workload": "DCF-MIC, 16 contiguous intervals, bitWidthIn=24, bitWidthOut=32, 2^22 elements",
                   "elements_per_s": N / wall, "interval_evals_per_s": N * I / wall, "kernel_ms": dev_ms,
                   "lookups_per_element": lookups, "lds_frac": lds_frac(lookups * N / (dev_ms * 1e-3))}
@@ -537,7 +569,8 @@ def piecewise_leg(reps: int = 3):
     gs = W.grotto_keys()
     lv = gs.n - 6
     pts = gs.N << gs.n
-    lookups = gs.N * ((1 << lv) - 1) * (LDS_COST[0] + 2 * LDS_COST[1])  # a schedule + two blocks per inner node
+    # per inner node: a key schedule and both children's blocks
+    lookups = gs.N * ((1 << lv) - 1) * (LK_SCHED + LK_BLOCK + LK_CTR2)
     full = {"workload": "full-domain Grotto [x < alpha] shares, bitWidthIn=16, 4096 keys x 2^16 points, party 0"}
     for name, packed, out in (("elements", 0, torch.empty(pts, dtype=torch.int16, device="cuda:0")),
                               ("packed_bits", 1, torch.empty(pts // 64, dtype=torch.int64, device="cuda:0"))):
@@ -763,6 +796,7 @@ def main():
             "kernel": "dcf_eval_queue_kernel<uint32_t, 8, true>",
             "kernel_ms": kernel_ms,
             "lookups_per_eval": n * LOOKUPS_PER_LEVEL,
+            "survey_lookups_per_eval": n * SURVEY_LOOKUPS_PER_LEVEL,
             "peak_source": LDS_PEAK["source"],
             "peak_words_per_clk_per_cu": live["words_per_clk_per_cu"],
             "peak_recorded": recorded_lds_peak(),
@@ -774,12 +808,13 @@ def main():
             "hbm_frac": hbm_gbs / HBM_PEAK_GBS,
             "note": "bound: the LDS pipe that feeds the T-table AES. peak: the highest rate at which the LDS "
                     "probe (probe/lds_probe.hip) gets 4-byte reads through on this box, in the same run. "
-                    "achieved: algorithmic lookups, per level one key schedule (40) and two blocks (320). "
-                    "The kernel issues fewer (issued_lookups_per_eval, rocprofv3 SQ_INSTS_LDS): the second "
-                    "block of a level shares round-1 and round-2 lookups with the first and its value block "
-                    "needs one last-round column, so frac can pass 1; issued_frac is the LDS pipe's actual "
-                    "load against the probe's ceiling. valu_issued_*: rocprofv3 SQ_INSTS_VALU against the "
-                    "VALU lane peak",
+                    "achieved: algorithmic lookups per level, 326 = key schedule 40 + the seed block 160 + the "
+                    "value block 126 (its plaintext differs from the seed block's in byte 0 of words 0 and 2, "
+                    "so it shares 14 lookups of round 1 and 8 of round 2, and only its first output column "
+                    "is used, which saves 12 of the last round). The survey's figure charges both blocks in "
+                    "full (360 per level, survey_lookups_per_eval) and is not a lower bound. "
+                    "issued_lookups_per_eval: rocprofv3 SQ_INSTS_LDS x 64 per eval, issued_frac its rate "
+                    "against the same peak. valu_issued_*: rocprofv3 SQ_INSTS_VALU against the VALU lane peak",
         },
         "lds_probe": probe,
     }

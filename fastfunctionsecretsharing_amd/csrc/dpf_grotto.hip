@@ -15,23 +15,6 @@
 
 namespace fss {
 
-#define FSS_KERNEL_PROLOGUE                 \
-    __shared__ uint32_t lds[kLdsTableWords]; \
-    aes_fill_lds(lds);                      \
-    __syncthreads();                        \
-    const AesCtx c = aes_ctx(lds);
-
-// the same for kernels that often run in small workgroups (aes_fill_lds_staged)
-#define FSS_KERNEL_PROLOGUE_STAGED                    \
-    __shared__ __align__(16) uint32_t lds[kLdsTableWords]; \
-    aes_fill_lds_staged(lds);                         \
-    __syncthreads();                                  \
-    const AesCtx c = aes_ctx(lds);
-
-#define FSS_GRID_LOOP(i, count)                                                             \
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < (count); \
-         i += (size_t)gridDim.x * blockDim.x)
-
 // ------------------------------------------------------------------ DPF ----
 
 template <typename T>

@@ -32,6 +32,11 @@
 //      ds_read_b32 at immediate offsets from one address register, which
 //      waits for all of them before it ends (its outputs are early-clobber
 //      and all consumed), so no VALU work sits between the reads
+//   8  as 0, software-pipelined: the reads of group k + 1 are issued before
+//      the values of group k are folded, so a wave never drains its reads
+//      (K = 16 per group, 32 in flight)
+//   9  as 8 without OPAQUE: the compiler merges the reads two per
+//      ds_read2st64_b32
 //
 // fss_lds_probe() runs a layout on `blocks` workgroups of 1024 lanes (one per
 // CU at this LDS size) and returns the kernel time and the number of 4-byte
@@ -100,6 +105,58 @@ __global__ __launch_bounds__(1024) void lds_probe_kernel(uint32_t *out, int iter
     out[blockIdx.x * blockDim.x + threadIdx.x] = acc;
 }
 
+// Layouts 8 and 9: groups of K reads, group k + 1 issued before group k's
+// values are folded (see above).  Same address bounds as lds_probe_kernel.
+template <bool OPAQUE, int K = 16>
+__global__ __launch_bounds__(1024) void lds_probe_pipelined_kernel(uint32_t *out, int iters, uint32_t seed)
+{
+    constexpr uint32_t PITCH = 256;
+    __shared__ __align__(16) uint32_t lds[kBytes / 4];
+    for (uint32_t i = threadIdx.x; i < kBytes / 4; i += blockDim.x) lds[i] = i * 2654435761u ^ seed;
+    __syncthreads();
+    constexpr uint32_t groups = kBytes / (K * PITCH);
+    constexpr uint32_t span   = groups * K * PITCH;
+    const uint32_t     L      = threadIdx.x & 63;
+    const uint32_t     slot   = OPAQUE ? 4 * (L & 31) : 4 * L;
+    const uint32_t     g0     = (threadIdx.x * 2654435761u ^ seed) % groups;
+    uint32_t           base   = g0 * K * PITCH + slot;
+    uint32_t           acc    = 0;
+    auto               load   = [&](uint32_t v[K]) {
+        if (OPAQUE)
+        {
+            uint32_t a[K];
+#pragma unroll
+            for (int i = 0; i < K; i++)
+            {
+                a[i] = base + i * PITCH;
+                asm volatile("" : "+v"(a[i]));
+            }
+#pragma unroll
+            for (int i = 0; i < K; i++) v[i] = *(const uint32_t *)((const char *)lds + a[i]);
+        }
+        else
+        {
+            const uint32_t *p = (const uint32_t *)((const char *)lds + base);
+#pragma unroll
+            for (int i = 0; i < K; i++) v[i] = p[i * (PITCH / 4)];
+        }
+        base += K * PITCH;
+        if (base >= span) base -= span;
+    };
+    uint32_t prev[K];
+    load(prev);
+    for (int it = 0; it < iters; it++)
+    {
+        uint32_t cur[K];
+        load(cur);
+#pragma unroll
+        for (int i = 0; i < K; i++) acc ^= prev[i], prev[i] = cur[i];
+    }
+#pragma unroll
+    for (int i = 0; i < K; i++) acc ^= prev[i];
+    out[blockIdx.x * blockDim.x + threadIdx.x] = acc;
+}
+
 // Layout 7: the 32 reads of a group as one asm block (see above); the LDS
 // address of the table comes from an address-space cast of its pointer.
 __global__ __launch_bounds__(1024) void lds_probe_asm_kernel(uint32_t *out, int iters, uint32_t seed)
@@ -156,7 +213,7 @@ __global__ __launch_bounds__(1024) void lds_probe_asm_kernel(uint32_t *out, int 
 extern "C" __attribute__((visibility("default"))) int fss_lds_probe(int layout, int blocks, int iters, double *ms,
                                                                     double *words)
 {
-    if (layout < 0 || layout > 7 || blocks <= 0 || iters <= 0) return -1;
+    if (layout < 0 || layout > 9 || blocks <= 0 || iters <= 0) return -1;
     uint32_t *out = nullptr;
     if (hipMalloc(&out, (size_t)blocks * 1024 * sizeof(uint32_t)) != hipSuccess) return -1;
     hipEvent_t e0, e1;
@@ -171,6 +228,8 @@ extern "C" __attribute__((visibility("default"))) int fss_lds_probe(int layout, 
             case 4: lds_probe_kernel<16, 1024, false, false><<<blocks, 1024>>>(out, it, 7u); break;
             case 5: lds_probe_kernel<4, 256, false, false><<<blocks, 1024>>>(out, it, 7u); break;
             case 7: lds_probe_asm_kernel<<<blocks, 1024>>>(out, it, 7u); break;
+            case 8: lds_probe_pipelined_kernel<true><<<blocks, 1024>>>(out, it, 7u); break;
+            case 9: lds_probe_pipelined_kernel<false><<<blocks, 1024>>>(out, it, 7u); break;
             default: lds_probe_kernel<4, 256, true, true, 8><<<blocks, 1024>>>(out, it, 7u); break;
         }
     };
@@ -183,7 +242,10 @@ extern "C" __attribute__((visibility("default"))) int fss_lds_probe(int layout, 
     (void)hipEventElapsedTime(&t, e0, e1);
     const int w = layout == 3 ? 2 : (layout == 4 ? 4 : 1);
     *ms    = t;
-    *words = (double)blocks * 1024.0 * (double)iters * (layout == 2 || layout == 6 ? 8.0 : 32.0) * w;
+    // reads per lane and iteration: 8 (layouts 2, 6), 16 (8, 9; the pipelined
+    // kernels' one extra group is not counted), 32 (the others)
+    const double perIter = (layout == 2 || layout == 6) ? 8.0 : (layout >= 8 ? 16.0 : 32.0);
+    *words = (double)blocks * 1024.0 * (double)iters * perIter * w;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipFree(out);
