@@ -140,8 +140,9 @@ def parse():
     ap.add_argument("--no-dpf", action="store_true", help="skip the plain DPF leg")
     ap.add_argument("--no-piecewise", action="store_true", help="skip the MIC + spline + full-domain Grotto leg")
     ap.add_argument("--no-ottt", action="store_true", help="skip the OTTT leg")
+    ap.add_argument("--no-small", action="store_true", help="skip the small-call leg")
     ap.add_argument("--only", default="", help="comma-separated legs to run after the headline step "
-                    "(dpf, piecewise, ottt, ragged, serving, cpu); default: all not skipped")
+                    "(dpf, piecewise, ottt, small, ragged, serving, cpu); default: all not skipped")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after timing, run one device-to-device copy of this many MiB (counter calibration)")
     return ap.parse_args()
@@ -403,6 +404,47 @@ def dpf_leg(log2n: int, reps: int = 10):
     return {"workload": "plain DPF eval, bitWidthIn=32, bitWidthOut=32, groupSize=1, 2^%d elements, party 0" % log2n,
             "parity_ok": ok, "evals_per_s": N / wall, "kernel_ms": dev_ms, "lookups_per_eval": lookups,
             "lds_frac": lds_frac(lookups * N / (dev_ms * 1e-3))}
+
+
+def small_calls_leg(sizes=(1024, 16384), reps=50):
+    """Per-call kernel time of small calls, which fill only part of the GPU: DCF and DPF eval at
+    bitWidthIn = bitWidthOut = 32, and the degree-2 spline of 16 intervals at bitWidthIn = 24,
+    bitWidthOut = 32; one party, HIP events over `reps` back-to-back calls on one stream."""
+    import torch
+
+    from fastfunctionsecretsharing_amd import _lib, workloads as W
+
+    lib = _lib.load()
+    dev = torch.device("cuda:0")
+    holder, sp = W.stream_ptr()
+    stream = torch.cuda.current_stream()
+    res = {}
+    for N in sizes:
+        gen = torch.Generator(device=dev).manual_seed(N)
+        for name, op, fn in (("dcf", _lib.BATCH_DCF_EVAL, lib.FastFss_cuda_dcfEval),
+                             ("dpf", _lib.BATCH_DPF_EVAL, lib.FastFss_cuda_dpfEval)):
+            g = W.make_group(op, 32, N, gen, dev, sp)
+            out = torch.empty(N, dtype=torch.int32, device=dev)
+
+            def call():
+                rc = fn(out.data_ptr(), 4 * N, g.masked.data_ptr(), 4 * N, g.key.data_ptr(), g.key_size,
+                        g.seeds[0].data_ptr(), 16 * N, 0, 32, 32, 1, 4, N, None, 0, sp)
+                assert rc == 0, rc
+
+            call()
+            _, ms = _timed(stream, call, reps)
+            res["%s_n32_%d" % (name, N)] = {"elements": N, "kernel_ms_per_call": ms}
+            del g, out
+        w = W.piecewise_workload(N=N, mic=False)
+        cache = W.spline_cache(w)
+        out = torch.empty(N, dtype=torch.int32, device=dev)
+        W.spline_eval(w, 0, out, w.e, w.beta, cache, sp)
+        _, ms = _timed(stream, lambda: W.spline_eval(w, 0, out, w.e, w.beta, cache, sp), reps)
+        res["spline_n24_%d" % N] = {"elements": N, "kernel_ms_per_call": ms}
+        del w, cache, out
+    del holder
+    torch.cuda.empty_cache()
+    return res
 
 
 LDS_PROBE_LAYOUTS = {
@@ -694,6 +736,7 @@ def main():
         legs = set(a.only.split(","))
         a.no_dpf, a.no_piecewise, a.no_ottt = "dpf" not in legs, "piecewise" not in legs, "ottt" not in legs
         a.no_ragged, a.no_serve, a.no_cpu = "ragged" not in legs, "serving" not in legs, "cpu" not in legs
+        a.no_small = "small" not in legs
     if a.gpus != 1:
         print(json.dumps({"error": "this benchmark measures one GPU; multi-GPU runs are not supported",
                           "n_gpus": a.gpus}))
@@ -830,6 +873,9 @@ def main():
     if not a.no_ottt:
         res["ottt"] = ottt_leg()
         progress("ottt done")
+    if not a.no_small:
+        res["small_calls"] = small_calls_leg()
+        progress("small calls done")
     if not a.no_ragged:
         res["ragged"] = ragged_leg(a.ragged_groups)
         progress("ragged done")

@@ -100,6 +100,13 @@ SIGNATURES = {
     "FastFss_hip_serverDrain": ("p", "int"),
     "FastFss_hip_serverStats": ("pp", "int"),
     "FastFss_hip_serverDestroy": ("p", None),
+    "FastFss_hip_keyStoreCreate": ("ps", "int"),
+    "FastFss_hip_keyStoreDestroy": ("p", "int"),
+    "FastFss_hip_keyStorePut": ("ppspp", "int"),
+    "FastFss_hip_keyStoreReserve": ("pspp", "int"),
+    "FastFss_hip_keyStoreGet": ("pqpp", "int"),
+    "FastFss_hip_keyStoreRelease": ("pqp", "int"),
+    "FastFss_hip_keyStoreStats": ("ppppp", "int"),
 }
 
 

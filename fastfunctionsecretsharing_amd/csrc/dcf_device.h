@@ -307,57 +307,10 @@ __device__ __forceinline__ void dcf_eval_range(const AesCtx &c, T acc[R], const 
     }
 }
 
-// The R correction words at p[0 .. R) of a walk that uses the first cnt of
-// them.  With `wide` (the caller guarantees that all R lie inside the key)
-// and 4- or 8-byte elements they come in 16-byte loads, so that a lane's
-// 32 bytes touch their one or two cache lines in two requests instead of one
-// request per word; otherwise word by word, the last used one standing in for
-// the unused ones, all inside the element's row.
-template <typename T, int R>
-__device__ __forceinline__ void slot_corrections(T vc[R], const T *p, int cnt, bool wide)
-{
-    constexpr int BYTES = R * (int)sizeof(T);
-    if constexpr (sizeof(T) >= 4 && BYTES % 16 == 0)
-    {
-        if (wide)
-        {
-            typedef uint32_t u32x4_a4 __attribute__((ext_vector_type(4), aligned(4)));
-            uint32_t w[BYTES / 4];
-#pragma unroll
-            for (int q = 0; q < BYTES / 16; q++)
-            {
-                const u32x4_a4 v = ((const u32x4_a4 *)p)[q];
-                w[4 * q] = v.x, w[4 * q + 1] = v.y, w[4 * q + 2] = v.z, w[4 * q + 3] = v.w;
-            }
-#pragma unroll
-            for (int r = 0; r < R; r++)
-            {
-                if constexpr (sizeof(T) == 4)
-                    vc[r] = (T)w[r];
-                else
-                    vc[r] = (T)((uint64_t)w[2 * r] | ((uint64_t)w[2 * r + 1] << 32));
-            }
-            return;
-        }
-    }
-#pragma unroll
-    for (int r = 0; r < R; r++) vc[r] = p[r < cnt ? r : cnt - 1];
-}
-
 // dcf_eval_range for a compile-time output width OB = ceil(bitWidthOut/8):
 // the slots' conversion shares one key schedule per level across its AES
 // windows and extracts bytes at static positions (SlotConv).  Requires
-// cnt * OB <= 32 and cnt <= R, and for OB = 4 or 8 every window counter
-// 16w/OB below 256 (slot_conv_fast).  Same results as dcf_eval_range for
-// acc[0 .. cnt); acc[cnt .. R) are left with values of no meaning (their
-// correction words are read at the last slot, in bounds, so that no slot
-// needs a branch).
-//
-// Code size decides the speed here: the walk runs 17 and more times per
-// element, and a copy of the level loop per window count overflowed the
-// instruction cache.  So the level's PRG is written once, only the
-// conversion is switched on the walk's window count (wave-uniform), and the
-// leaf's conversion is the loop's last pass, which skips the PRG.
+// cnt * OB <= 32 and cnt <= R.  Same results as dcf_eval_range.
 template <typename T, int OB, int R>
 __device__ __forceinline__ void dcf_eval_slots(const AesCtx &c, T acc[R], const uint8_t *sCW, const T *tail, T x,
                                                const uint32_t seed[4], int party, int n, int g, int j0, int cnt)
@@ -365,58 +318,51 @@ __device__ __forceinline__ void dcf_eval_slots(const AesCtx &c, T acc[R], const 
     constexpr int NWORDS = (R * OB + 3) / 4;
     SlotConv<OB, 3> cv;
     cv.setup(g, j0, cnt);
-    const int kw = cv.windows();
-    uint32_t  s[4] = {seed[0] & ~1u, seed[1], seed[2], seed[3]};
-    uint32_t  t    = (uint32_t)party;
-    x              = mod_bits<T>(x, n);
+    uint32_t s[4] = {seed[0] & ~1u, seed[1], seed[2], seed[3]};
+    uint32_t t    = (uint32_t)party;
+    x             = mod_bits<T>(x, n);
 #pragma unroll
     for (int r = 0; r < R; r++) acc[r] = 0;
     const T tl = tail[(size_t)n * g], tr = tail[(size_t)n * g + 1];
-    for (int i = 0;; i++)
+    for (int i = 0; i < n; i++)
     {
-        // levels 0..n-1: expand the seed and convert the taken child's value
-        // block; pass n: convert the leaf seed against lastCW
-        uint32_t key[4];
-        uint32_t tn  = t;
-        const T *cor = tail + (size_t)n * g + 2 + j0;
-        if (i < n)
-        {
-            const uint32_t xi = (uint32_t)(x >> (n - 1 - i)) & 1u;
-            uint32_t       cw[4];
-            load16(cw, sCW + 16 * i);
-            uint32_t blk[2][4];
-            aes_encrypt_dcf_pair(c, s, xi, blk);
-            const uint32_t mk = 0u - t;
-            tn                = blk[0][0] & 1u;
-            s[0]              = (blk[0][0] & ~1u) ^ (cw[0] & mk);
-            s[1]              = blk[0][1] ^ (cw[1] & mk);
-            s[2]              = blk[0][2] ^ (cw[2] & mk);
-            s[3]              = blk[0][3] ^ (cw[3] & mk);
-            tn ^= (uint32_t)(((xi ? tr : tl) >> i) & 1) & t;
-            key[0] = blk[1][0], key[1] = blk[1][1], key[2] = blk[1][2], key[3] = blk[1][3];
-            cor = tail + (size_t)i * g + j0;
-        }
-        else
-            key[0] = s[0], key[1] = s[1], key[2] = s[2], key[3] = s[3];
-        // the pass's correction words, all in flight during the conversion
-        T vc[R];
-        slot_corrections<T, R>(vc, cor, cnt, i < n && g >= R);
+        const uint32_t xi = (uint32_t)(x >> (n - 1 - i)) & 1u;
+        uint32_t       cw[4];
+        load16(cw, sCW + 16 * i);
+        uint32_t blk[2][4];
+        aes_encrypt_dcf_pair(c, s, xi, blk);
+        const uint32_t mk = 0u - t;
+        uint32_t       tn = blk[0][0] & 1u;
+        s[0]              = (blk[0][0] & ~1u) ^ (cw[0] & mk);
+        s[1]              = blk[0][1] ^ (cw[1] & mk);
+        s[2]              = blk[0][2] ^ (cw[2] & mk);
+        s[3]              = blk[0][3] ^ (cw[3] & mk);
+        tn ^= (uint32_t)(((xi ? tr : tl) >> i) & 1) & t;
         uint32_t words[NWORDS];
-        switch (kw)
-        {
-        case 0: cv.template fill<NWORDS, 0>(c, key, words); break;
-        case 1: cv.template fill<NWORDS, 1>(c, key, words); break;
-        case 2: cv.template fill<NWORDS, 2>(c, key, words); break;
-        default: cv.template fill<NWORDS, 3>(c, key, words); break;
-        }
+        cv.template fill<NWORDS>(c, blk[1], words);
+        const T *vcw = tail + (size_t)i * g + j0;
 #pragma unroll
         for (int r = 0; r < R; r++)
         {
-            T term = (T)((T)SlotConv<OB, 3>::template at<NWORDS>(words, r) + (t ? vc[r] : (T)0));
+            if (r < cnt)
+            {
+                T term = (T)((T)SlotConv<OB, 3>::template at<NWORDS>(words, r) + (t ? vcw[r] : (T)0));
+                acc[r] = (T)(acc[r] + neg_if<T>(term, party));
+            }
+        }
+        t = tn;
+    }
+    uint32_t words[NWORDS];
+    cv.template fill<NWORDS>(c, s, words);
+    const T *last = tail + (size_t)n * g + 2 + j0;
+#pragma unroll
+    for (int r = 0; r < R; r++)
+    {
+        if (r < cnt)
+        {
+            T term = (T)((T)SlotConv<OB, 3>::template at<NWORDS>(words, r) + (t ? last[r] : (T)0));
             acc[r] = (T)(acc[r] + neg_if<T>(term, party));
         }
-        if (i == n) break;
-        t = tn;
     }
 }
 

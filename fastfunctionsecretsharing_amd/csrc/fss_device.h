@@ -271,6 +271,31 @@ __device__ __forceinline__ void aes_encrypt(const AesCtx &c, const uint32_t key[
     for (int b = 0; b < NB; b++) aes_last(c, st[b], k);
 }
 
+// The first `nb` (<= NB, wave-uniform) of NB blocks under one key schedule;
+// blocks nb..NB-1 are left as they are.
+template <int NB>
+__device__ __forceinline__ void aes_encrypt_upto(const AesCtx &c, const uint32_t key[4], uint32_t st[NB][4], int nb)
+{
+    uint32_t k[4] = {key[0], key[1], key[2], key[3]};
+#pragma unroll
+    for (int b = 0; b < NB; b++)
+        if (b < nb) st[b][0] ^= k[0], st[b][1] ^= k[1], st[b][2] ^= k[2], st[b][3] ^= k[3];
+    const uint32_t rcon[10] = {0x01, 0x02, 0x04, 0x08, 0x10, 0x20, 0x40, 0x80, 0x1b, 0x36};
+#pragma unroll
+    for (int r = 0; r < 9; r++)
+    {
+        aes_next_key(c, k, rcon[r]);
+        const uint32_t *h = k;
+#pragma unroll
+        for (int b = 0; b < NB; b++)
+            if (b < nb) aes_round(c, st[b], h);
+    }
+    aes_next_key(c, k, rcon[9]);
+#pragma unroll
+    for (int b = 0; b < NB; b++)
+        if (b < nb) aes_last(c, st[b], k);
+}
+
 // E independent keys, NB blocks each, rounds interleaved across keys so the
 // lookups of all E*NB states are in flight together.
 template <int E, int NB>
@@ -349,15 +374,13 @@ __device__ __forceinline__ void aes_encrypt_lowpt(const AesCtx &c, const uint32_
     for (int q = 0; q < NB; q++) aes_last(c, st[q], k);
 }
 
-// aes_encrypt_lowpt for the conversion windows: NB blocks with plaintexts
-// (0, 0, p2[q], 0), every p2[q] < 256.  The blocks differ only in byte 0 of
-// word 2, so round 1 costs 15 shared lookups plus 1 per block.  NB is a
-// compile-time count so that every round of every block sits in one basic
-// block and the lookups of all blocks and the key schedule are in flight
-// together.
+// aes_encrypt_lowpt for the conversion windows: plaintexts (0, 0, p2[q], 0)
+// with every p2[q] < 256, only the first `nb` (<= NB, wave-uniform) blocks.
+// The blocks differ only in byte 0 of word 2, so round 1 costs 15 shared
+// lookups plus 1 per block.
 template <int NB>
-__device__ __forceinline__ void aes_encrypt_win(const AesCtx &c, const uint32_t key[4], const uint32_t p2[NB],
-                                                uint32_t st[NB][4])
+__device__ __forceinline__ void aes_encrypt_win_upto(const AesCtx &c, const uint32_t key[4], const uint32_t p2[NB],
+                                                     uint32_t st[NB][4], int nb)
 {
     uint32_t       k[4] = {key[0], key[1], key[2], key[3]};
     const uint32_t a0 = k[0], a1 = k[1], a2 = k[2], a3 = k[3];
@@ -369,19 +392,19 @@ __device__ __forceinline__ void aes_encrypt_win(const AesCtx &c, const uint32_t 
     const uint32_t r3 = xor3(te<0>(c, a3, 0), te<1>(c, a0, 1), xor3(te<2>(c, a1, 2), te<3>(c, a2, 3), k[3]));
 #pragma unroll
     for (int q = 0; q < NB; q++)
-        st[q][0] = r0, st[q][1] = r1, st[q][2] = r2 ^ te<0>(c, p2[q] ^ a2, 0), st[q][3] = r3;
+        if (q < nb) st[q][0] = r0, st[q][1] = r1, st[q][2] = r2 ^ te<0>(c, p2[q] ^ a2, 0), st[q][3] = r3;
 #pragma unroll
     for (int r = 1; r < 9; r++)
     {
         aes_next_key(c, k, rcon[r]);
 #pragma unroll
         for (int q = 0; q < NB; q++)
-            aes_round(c, st[q], k);
+            if (q < nb) aes_round(c, st[q], k);
     }
     aes_next_key(c, k, rcon[9]);
 #pragma unroll
     for (int q = 0; q < NB; q++)
-        aes_last(c, st[q], k);
+        if (q < nb) aes_last(c, st[q], k);
 }
 
 // The two blocks a DCF walk needs at one level (x = input bit).
@@ -501,11 +524,8 @@ struct ConvCtx
 // When OB * g > 16 the slots' bytes come from the AES windows w0 .. w0+nw-1
 // (window w = AES_value(0, 16w / OB)), all encrypted under one key schedule;
 // at most NW windows (the caller guarantees (j0*OB mod 16) + cnt*OB <= 16*NW).
-// windows() is 0 for the direct form (the bytes of the value block itself)
-// and nw otherwise; the caller passes it to fill() as the compile-time K, so
-// the encryption of all K windows is straight-line code.  fill() leaves the
-// bytes starting at slot j0 in words[], so slot j0 + r is bytes
-// [r*OB, (r+1)*OB) of words[] -- static positions.
+// fill() leaves the bytes starting at slot j0 in words[], so slot j0 + r is
+// bytes [r*OB, (r+1)*OB) of words[] -- static positions.
 template <int OB, int NW>
 struct SlotConv
 {
@@ -520,59 +540,47 @@ struct SlotConv
         b0              = start - 16 * w0;
     }
 
-    __device__ __forceinline__ int windows() const { return direct ? 0 : nw; }
-
-    // plaintext counters 16w/OB of the K windows
-    template <int K>
-    __device__ __forceinline__ void counters(uint32_t p2[K]) const
-    {
-#pragma unroll
-        for (int k = 0; k < K; k++) p2[k] = (uint32_t)((16 * (w0 + k)) / OB);
-    }
-
-    template <int NWORDS, int K>
+    template <int NWORDS>
     __device__ __forceinline__ void fill(const AesCtx &c, const uint32_t v[4], uint32_t words[NWORDS]) const
     {
-        static_assert(K >= 0 && K <= NW, "window count out of range");
-        uint32_t blk[K > 0 ? K : 1][4];
-        if constexpr (K == 0)
+        static_assert(NWORDS + 4 <= 4 * (NW + 1), "window buffer too small");
+        uint32_t w[NW + 1][4];
+        if (direct)
         {
 #pragma unroll
-            for (int i = 0; i < 4; i++) blk[0][i] = v[i];
-        }
-        // plaintexts (0, 0, 16w/OB, 0): the shared-round-1 form for 4- and
-        // 8-byte slots, whose counters the caller keeps below 256
-        // (slot_conv_fast), the general one for the other widths (with
-        // narrower slots the extra shared terms cost registers)
-        else if constexpr (OB == 4 || OB == 8)
-        {
-            uint32_t p2[K];
-            counters<K>(p2);
-            aes_encrypt_win<K>(c, v, p2, blk);
+            for (int i = 0; i < 4; i++) w[0][i] = v[i];
+#pragma unroll
+            for (int k = 1; k <= NW; k++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) w[k][i] = 0;
         }
         else
         {
+            uint32_t blk[NW][4];
+            // (only for 4- and 8-byte slots: with narrower slots the extra
+            // shared terms push the spline kernel past its register budget)
+            if ((OB == 4 || OB == 8) && (16 * (w0 + NW - 1)) / OB < 256)
+            {
+                uint32_t p2[NW];
 #pragma unroll
-            for (int k = 0; k < K; k++)
-                blk[k][0] = 0u, blk[k][1] = 0u, blk[k][2] = (uint32_t)((16 * (w0 + k)) / OB), blk[k][3] = 0u;
-            aes_encrypt<K>(c, v, blk);
+                for (int k = 0; k < NW; k++) p2[k] = (uint32_t)((16 * (w0 + k)) / OB);
+                aes_encrypt_win_upto<NW>(c, v, p2, blk, nw);
+            }
+            else
+            {
+#pragma unroll
+                for (int k = 0; k < NW; k++)
+                    blk[k][0] = 0u, blk[k][1] = 0u, blk[k][2] = (uint32_t)((16 * (w0 + k)) / OB), blk[k][3] = 0u;
+                aes_encrypt_upto<NW>(c, v, blk, nw);
+            }
+#pragma unroll
+            for (int k = 0; k < NW; k++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) w[k][i] = blk[k][i];
+#pragma unroll
+            for (int i = 0; i < 4; i++) w[NW][i] = 0;
         }
-        place<NWORDS, K>(blk, words);
-    }
-
-    // words[] from the K encrypted windows (K = 0: blk[0] is the value block
-    // itself): the window bytes shifted left by b0 (0..15), whole words by
-    // b0/4, then bytes by b0%4
-    template <int NWORDS, int K>
-    __device__ __forceinline__ void place(const uint32_t blk[K > 0 ? K : 1][4], uint32_t words[NWORDS]) const
-    {
-        static_assert(NWORDS + 4 <= 4 * (NW + 1), "window buffer too small");
-        constexpr int KB = K > 0 ? K : 1;
-        uint32_t      w[NW + 1][4];
-#pragma unroll
-        for (int k = 0; k <= NW; k++)
-#pragma unroll
-            for (int i = 0; i < 4; i++) w[k][i] = k < KB ? blk[k < KB ? k : 0][i] : 0u;
+        // shift the window bytes left by b0 (0..15): whole words by b0/4, then bytes by b0%4
         const int      ws = b0 >> 2;
         const uint32_t bs = (uint32_t)(b0 & 3);
 #pragma unroll
@@ -607,14 +615,6 @@ struct SlotConv
         return v;
     }
 };
-
-// Whether SlotConv serves a g-slot group of OB-byte slots: with 4- or 8-byte
-// slots its windows use the shared-round-1 encryption, whose counters 16w/OB
-// must stay below 256 (g <= 256 slots).
-__host__ __device__ constexpr bool slot_conv_fast(int OB, long g)
-{
-    return !(OB == 4 || OB == 8) || OB * g <= 16 || (16 * ((OB * g - 1) / 16)) / OB < 256;
-}
 
 // ---------------------------------------------------------------------------
 // work distribution

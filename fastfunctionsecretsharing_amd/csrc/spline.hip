@@ -146,10 +146,7 @@ static void spline_launch(int grid, int block, hipStream_t st, void *out, const 
 {
     const int ob = (m + 7) / 8;
     // (64-bit elements with 2-byte outputs keep the generic kernel: its register footprint is smaller)
-    const int fast = (2 * (degree + 1) * ob <= 32 && !(sizeof(T) == 8 && ob == 2) &&
-                      slot_conv_fast(ob, (long)(I * (size_t)(degree + 1))))
-                         ? ob
-                         : 0;
+    const int fast = (2 * (degree + 1) * ob <= 32 && !(sizeof(T) == 8 && ob == 2)) ? ob : 0;
 #define FSS_SPLINE_OB(K)                                                                                            \
     case K:                                                                                                         \
         if constexpr (K <= (int)sizeof(T))                                                                          \

@@ -24,20 +24,16 @@ namespace fss {
 // point instead of taking it from its predecessor's walk (the same value), and
 // the bpe partial sums are added across the lanes, whose results are equal to
 // the one-lane walk's mod 2^m.
-template <typename T, int OB, int RS>
-__device__ __forceinline__ void spline_walk(const AesCtx &c, T acc[RS], const uint8_t *sc, const T *tail, T pt,
+template <typename T, int OB>
+__device__ __forceinline__ void spline_walk(const AesCtx &c, T *acc, const uint8_t *sc, const T *tail, T pt,
                                             const uint32_t s[4], int party, int n, int m, int g, int j0, int cnt)
 {
     if constexpr (OB == 0)
-        dcf_eval_range<T, RS>(c, acc, sc, tail, pt, s, party, n, m, g, j0, cnt);
+        dcf_eval_range<T, 16>(c, acc, sc, tail, pt, s, party, n, m, g, j0, cnt);
     else
-        dcf_eval_slots<T, OB, RS>(c, acc, sc, tail, pt, s, party, n, g, j0, cnt);
+        dcf_eval_slots<T, OB, (32 / OB < 16 ? 32 / OB : 16)>(c, acc, sc, tail, pt, s, party, n, g, j0, cnt);
 }
 
-// The walks of one element's interval block [q0, q1), in order: the start
-// point x-1-L_q of an interval not linked to its predecessor (always for q0),
-// then the end point x-1-(R_q+1) of every interval.  One walk call site per
-// loop iteration keeps a single copy of each conversion width's walk code.
 template <typename T, int OB, bool SPLIT>
 __device__ __forceinline__ void spline_eval_body(const AesCtx &c, T *out, const T *x, const uint8_t *key, const T *sE,
                                                  const T *sBeta, const uint8_t *seed, int party, const T *left,
@@ -65,25 +61,21 @@ __device__ __forceinline__ void spline_eval_body(const AesCtx &c, T *out, const 
         T w[8];
 #pragma unroll
         for (int j = 0; j < 8; j++) w[j] = 0;
-        size_t q         = q0;
-        bool   needStart = true; // the next walk is interval q's start point
-        while (q < q1)
+        for (size_t q = q0; q < q1; q++)
         {
-            const T    L  = mod_bits<T>(left[q], n), R = mod_bits<T>(right[q], n);
-            const T    Rp = mod_bits<T>((T)(R + 1), n);
-            const bool start = needStart;
-            const bool link  = !start && (q + 1 < q1) && Rp == mod_bits<T>(left[q + 1], n);
-            T          acc[RS];
-            spline_walk<T, OB, RS>(c, acc, sc, tail, start ? (T)(xv - 1 - L) : (T)(xv - 1 - Rp), s, party, n, m,
-                                   (int)g, (int)(q * C), link ? 2 * C : C);
-            if (start)
+            const T L  = mod_bits<T>(left[q], n), R = mod_bits<T>(right[q], n);
+            const T Rp = mod_bits<T>((T)(R + 1), n);
+            T       acc[RS];
+            if (!(q > q0 && mod_bits<T>((T)(right[q - 1] + 1), n) == L))
             {
+                spline_walk<T, OB>(c, acc, sc, tail, (T)(xv - 1 - L), s, party, n, m, (int)g, (int)(q * C), C);
 #pragma unroll
                 for (int j = 0; j < 8; j++)
                     if (j < C) w[j] = (T)(w[j] - acc[j < RS ? j : 0]);
-                needStart = false;
-                continue;
             }
+            const bool link = (q + 1 < q1) && mod_bits<T>((T)(right[q] + 1), n) == mod_bits<T>(left[q + 1], n);
+            spline_walk<T, OB>(c, acc, sc, tail, (T)(xv - 1 - Rp), s, party, n, m, (int)g, (int)(q * C),
+                               link ? 2 * C : C);
             const T cx = (T)((int)(xv > L) - (int)(xv > Rp));
 #pragma unroll
             for (int j = 0; j < 8; j++)
@@ -104,8 +96,6 @@ __device__ __forceinline__ void spline_eval_body(const AesCtx &c, T *out, const 
                     w[j] = (T)(w[j] - v);
                 }
             }
-            needStart = !link;
-            q++;
         }
         T r = 0, pw = 1;
 #pragma unroll
