@@ -236,7 +236,7 @@ static int validate(const FastFssBatchGroup &q)
 {
     const size_t n = q.bitWidthIn, m = q.bitWidthOut, es = q.elementSize, N = q.elementNum;
     if (q.op != FAST_FSS_BATCH_DCF_EVAL && q.op != FAST_FSS_BATCH_DPF_EVAL) return FAST_FSS_RUNTIME_ERROR;
-    if (!elem_size_ok(es)) return FAST_FSS_INVALID_ELEMENT_SIZE_ERROR;
+    if (!elem_size_narrow(es)) return FAST_FSS_INVALID_ELEMENT_SIZE_ERROR;
     if (!(0 < n && n <= es * 8)) return FAST_FSS_INVALID_BITWIDTH_ERROR;
     if (!(0 < m && m <= es * 8)) return FAST_FSS_INVALID_BITWIDTH_ERROR;
     if (q.partyId != 0 && q.partyId != 1) return FAST_FSS_INVALID_PARTY_ID_ERROR;
@@ -276,7 +276,7 @@ using namespace fss;
 
 extern "C" FAST_FSS_API int FastFss_hip_batchSetDefaultTile(int op, size_t elementSize, uint32_t tileElems)
 {
-    if ((op != 0 && op != 1) || !elem_size_ok(elementSize) || tileElems == 0 || tileElems % 64 != 0)
+    if ((op != 0 && op != 1) || !elem_size_narrow(elementSize) || tileElems == 0 || tileElems % 64 != 0)
         return FAST_FSS_RUNTIME_ERROR;
     std::lock_guard<std::mutex> lock(g_tile_mu);
     g_tile[op][elementSize] = tileElems;
@@ -285,7 +285,7 @@ extern "C" FAST_FSS_API int FastFss_hip_batchSetDefaultTile(int op, size_t eleme
 
 extern "C" FAST_FSS_API uint32_t FastFss_hip_batchGetDefaultTile(int op, size_t elementSize)
 {
-    if ((op != 0 && op != 1) || !elem_size_ok(elementSize)) return 0;
+    if ((op != 0 && op != 1) || !elem_size_narrow(elementSize)) return 0;
     std::lock_guard<std::mutex> lock(g_tile_mu);
     return g_tile[op][elementSize];
 }

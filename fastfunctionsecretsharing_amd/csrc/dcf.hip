@@ -443,6 +443,11 @@ extern "C" FAST_FSS_API int FastFss_cuda_dcfKeyGen(void *key, size_t keyDataSize
     if (!aligned(key, 8) || !aligned(seed0, 8) || !aligned(seed1, 8)) return FAST_FSS_RUNTIME_ERROR;
     hipStream_t st = resolve_stream(cudaStreamPtr);
     const void *b  = betaDataSize ? beta : nullptr;
+    if (es > 8)
+    {
+        if (!aligned(alpha, 8) || (b && !aligned(b, 8))) return FAST_FSS_RUNTIME_ERROR;
+        return wide_dcf_keygen(st, key, alpha, b, seed0, seed1, (int)n, (int)m, g, es, N);
+    }
     if (g == 1)
     {
         uint32_t *counter = queue_counter_acquire(st);
@@ -487,6 +492,11 @@ extern "C" FAST_FSS_API int FastFss_cuda_dcfEval(void *sharedOut, size_t sharedO
     if (N == 0) return FAST_FSS_SUCCESS;
     if (!aligned(key, 8) || !aligned(seed, 8)) return FAST_FSS_RUNTIME_ERROR;
     hipStream_t       st = resolve_stream(cudaStreamPtr);
+    if (es > 8)
+    {
+        if (!aligned(sharedOut, 8) || !aligned(maskedX, 8)) return FAST_FSS_RUNTIME_ERROR;
+        return wide_dcf_eval(st, sharedOut, maskedX, key, seed, partyId, (int)n, (int)m, g, es, N);
+    }
     const LaunchShape ls = shape_for(N);
     if (g == 1)
     {
@@ -515,7 +525,7 @@ extern "C" FAST_FSS_API int FastFss_hip_dcfEvalNoAlloc(void *sharedOut, size_t s
 {
     (void)cache;
     const size_t n = bitWidthIn, m = bitWidthOut, g = groupSize, es = elementSize, N = elementNum;
-    if (!elem_size_ok(es)) return FAST_FSS_INVALID_ELEMENT_SIZE_ERROR;
+    if (!elem_size_narrow(es)) return FAST_FSS_INVALID_ELEMENT_SIZE_ERROR;
     if (!(0 < n && n <= es * 8)) return FAST_FSS_INVALID_BITWIDTH_ERROR;
     if (!(0 < m && m <= es * 8)) return FAST_FSS_INVALID_BITWIDTH_ERROR;
     if (partyId != 0 && partyId != 1) return FAST_FSS_INVALID_PARTY_ID_ERROR;

@@ -657,6 +657,11 @@ extern "C" FAST_FSS_API int FastFss_cuda_dpfKeyGen(void *key, size_t keyDataSize
     if (!aligned(key, 8) || !aligned(seed0, 8) || !aligned(seed1, 8)) return FAST_FSS_RUNTIME_ERROR;
     hipStream_t st = resolve_stream(cudaStreamPtr);
     const void *b  = betaDataSize ? beta : nullptr;
+    if (es > 8)
+    {
+        if (!aligned(alpha, 8) || (b && !aligned(b, 8))) return FAST_FSS_RUNTIME_ERROR;
+        return wide_dpf_keygen(st, key, alpha, b, seed0, seed1, (int)n, (int)m, g, es, N);
+    }
     uint32_t *counter = queue_counter_acquire(st);
     if (!counter) return FAST_FSS_RUNTIME_ERROR;
     FSS_DISPATCH(es, dpf_keygen_kernel<T><<<FSS_SHAPE(N), 0, st>>>((uint8_t *)key, (const T *)alpha,
@@ -696,6 +701,11 @@ extern "C" FAST_FSS_API int FastFss_cuda_dpfEval(void *sharedOut, size_t sharedO
     if (N == 0) return FAST_FSS_SUCCESS;
     if (!aligned(key, 8) || !aligned(seed, 8)) return FAST_FSS_RUNTIME_ERROR;
     hipStream_t st = resolve_stream(cudaStreamPtr);
+    if (es > 8)
+    {
+        if (!aligned(sharedOut, 8) || !aligned(maskedX, 8)) return FAST_FSS_RUNTIME_ERROR;
+        return wide_dpf_eval(st, 0, sharedOut, maskedX, key, seed, partyId, (int)n, (int)m, g, es, N, nullptr, 0);
+    }
     // measured on MI355X (n = m = 32): the static split beats the work queue up
     // to 2^20 elements (2^16: 55 against 82 us, 2^20: 446 against 500 us); the
     // queue evens out the waves of large launches
@@ -744,7 +754,12 @@ extern "C" FAST_FSS_API int FastFss_cuda_dpfEvalAll(void *sharedOut, size_t shar
     if (N == 0) return FAST_FSS_SUCCESS;
     if (!aligned(key, 8) || !aligned(seed, 8)) return FAST_FSS_RUNTIME_ERROR;
     hipStream_t st = resolve_stream(cudaStreamPtr);
-    int         L, D;
+    if (es > 8)
+    {
+        if (!aligned(sharedOut, 8) || !aligned(maskedX, 8)) return FAST_FSS_RUNTIME_ERROR;
+        return wide_dpf_eval(st, 1, sharedOut, maskedX, key, seed, partyId, (int)n, (int)m, g, es, N, nullptr, 0);
+    }
+    int L, D;
     full_domain_split((int)n, N, L, D);
     FSS_DISPATCH(es, dpf_eval_all_dfs_kernel<T><<<FSS_SHAPE(N << L), 0, st>>>(
                          (T *)sharedOut, (const T *)maskedX, (const uint8_t *)key, (const uint8_t *)seed, partyId,
@@ -774,6 +789,11 @@ extern "C" FAST_FSS_API int FastFss_cuda_dpfEvalMulti(void *sharedOut, size_t sh
     if (N == 0 || P == 0) return FAST_FSS_SUCCESS;
     if (!aligned(key, 8) || !aligned(seed, 8)) return FAST_FSS_RUNTIME_ERROR;
     hipStream_t st = resolve_stream(cudaStreamPtr);
+    if (es > 8)
+    {
+        if (!aligned(sharedOut, 8) || !aligned(maskedX, 8) || !aligned(point, 8)) return FAST_FSS_RUNTIME_ERROR;
+        return wide_dpf_eval(st, 2, sharedOut, maskedX, key, seed, partyId, (int)n, (int)m, g, es, N, point, P);
+    }
     FSS_DISPATCH(es, dpf_eval_multi_kernel<T><<<FSS_SHAPE(N * P), 0, st>>>(
                          (T *)sharedOut, (const T *)maskedX, (const uint8_t *)key, (const uint8_t *)seed, partyId,
                          (const T *)point, P, (int)n, (int)m, (int)g, N));
@@ -796,6 +816,11 @@ extern "C" FAST_FSS_API int FastFss_cuda_grottoKeyGen(void *key, size_t keyDataS
     if (N == 0) return FAST_FSS_SUCCESS;
     if (!aligned(key, 8) || !aligned(seed0, 8) || !aligned(seed1, 8)) return FAST_FSS_RUNTIME_ERROR;
     hipStream_t st = resolve_stream(cudaStreamPtr);
+    if (es > 8)
+    {
+        if (!aligned(alpha, 8)) return FAST_FSS_RUNTIME_ERROR;
+        return wide_grotto_keygen(st, key, alpha, seed0, seed1, (int)n, es, N);
+    }
     FSS_DISPATCH(es, grotto_keygen_kernel<T><<<FSS_SHAPE(N), 0, st>>>(
                          (uint8_t *)key, (const T *)alpha, (const uint8_t *)seed0, (const uint8_t *)seed1, (int)n, N));
     return launch_status();
@@ -823,6 +848,12 @@ static int grotto_eval_launch(void *out, size_t outSize, const void *x, size_t x
     if (N == 0) return FAST_FSS_SUCCESS;
     if (!aligned(key, 8) || !aligned(seed, 8)) return FAST_FSS_RUNTIME_ERROR;
     hipStream_t st = resolve_stream(sp);
+    if (es > 8)
+    {
+        if (!aligned(out, 8) || !aligned(x, 8)) return FAST_FSS_RUNTIME_ERROR;
+        return wide_grotto_eval(st, 0, mode, out, nullptr, nullptr, x, key, seed, party, (int)n, es, N, nullptr,
+                                nullptr, 0, nullptr, 0);
+    }
     FSS_DISPATCH(es, grotto_eval_kernel<T><<<FSS_SHAPE(N), 0, st>>>(
                          (T *)out, (const T *)x, (const uint8_t *)key, (const uint8_t *)seed, party, (int)n, mode, N));
     return launch_status();
@@ -878,6 +909,18 @@ extern "C" FAST_FSS_API int FastFss_cuda_grottoMICEval(void *sharedBooleanOut, s
     if (N == 0 || I == 0) return FAST_FSS_SUCCESS;
     if (!aligned(key, 8) || !aligned(seed, 8)) return FAST_FSS_RUNTIME_ERROR;
     hipStream_t st = resolve_stream(cudaStreamPtr);
+    if (es > 8)
+    {
+        if (!aligned(sharedBooleanOut, 8) || !aligned(maskedX, 8) || !aligned(leftEndpoints, 8) ||
+            !aligned(rightEndpoints, 8))
+            return FAST_FSS_RUNTIME_ERROR;
+        void *bits = nullptr;
+        if (hipMallocAsync(&bits, N * I * sizeof(int), st) != hipSuccess) return FAST_FSS_RUNTIME_ERROR;
+        const int rc = wide_grotto_eval(st, 1, 1, sharedBooleanOut, nullptr, bits, maskedX, key, seed, partyId,
+                                        (int)n, es, N, leftEndpoints, rightEndpoints, I, nullptr, 0);
+        if (hipFreeAsync(bits, st) != hipSuccess) return FAST_FSS_RUNTIME_ERROR;
+        return rc;
+    }
     FSS_DISPATCH(es, grotto_mic_kernel<T><<<FSS_SHAPE(N), 0, st>>>(
                          (T *)sharedBooleanOut, (const T *)maskedX, (const uint8_t *)key, (const uint8_t *)seed,
                          partyId, (const T *)leftEndpoints, (const T *)rightEndpoints, I, (int)n, N));
@@ -910,6 +953,17 @@ extern "C" FAST_FSS_API int FastFss_cuda_grottoIntervalLutEval(
     if (!aligned(key, 8) || !aligned(seed, 8)) return FAST_FSS_RUNTIME_ERROR;
     hipStream_t st   = resolve_stream(cudaStreamPtr);
     void       *bits = nullptr;
+    if (es > 8)
+    {
+        if (!aligned(sharedOutE, 8) || !aligned(sharedOutT, 8) || !aligned(maskedX, 8) || !aligned(leftEndpoints, 8) ||
+            !aligned(rightEndpoints, 8) || !aligned(lookUpTable, 8))
+            return FAST_FSS_RUNTIME_ERROR;
+        if (hipMallocAsync(&bits, N * I * sizeof(int), st) != hipSuccess) return FAST_FSS_RUNTIME_ERROR;
+        const int rc = wide_grotto_eval(st, 2, 1, sharedOutE, sharedOutT, bits, maskedX, key, seed, partyId, (int)n,
+                                        es, N, leftEndpoints, rightEndpoints, I, lookUpTable, lutNum);
+        if (hipFreeAsync(bits, st) != hipSuccess) return FAST_FSS_RUNTIME_ERROR;
+        return rc;
+    }
     if (hipMallocAsync(&bits, N * I * es, st) != hipSuccess) return FAST_FSS_RUNTIME_ERROR;
     FSS_DISPATCH(es, grotto_lut_kernel<T><<<FSS_SHAPE(N), 0, st>>>(
                          (T *)sharedOutE, (T *)sharedOutT, (T *)bits, (const T *)maskedX, (const uint8_t *)key,
@@ -929,7 +983,7 @@ extern "C" FAST_FSS_API int FastFss_hip_grottoEvalFullDomain(void *sharedOut, si
                                                              size_t elementNum, void *hipStreamPtr)
 {
     const size_t n = bitWidthIn, es = elementSize, N = elementNum;
-    if (!elem_size_ok(es)) return FAST_FSS_INVALID_ELEMENT_SIZE_ERROR;
+    if (!elem_size_narrow(es)) return FAST_FSS_INVALID_ELEMENT_SIZE_ERROR;
     if (!(6 <= n && n <= es * 8 && n <= 40)) return FAST_FSS_INVALID_BITWIDTH_ERROR;
     if (mode < FAST_FSS_GROTTO_FULL_LT || mode > FAST_FSS_GROTTO_FULL_EQ || (packed != 0 && packed != 1))
         return FAST_FSS_RUNTIME_ERROR;

@@ -74,7 +74,15 @@ inline int launch_status()
     return hipPeekAtLastError() == hipSuccess ? FAST_FSS_SUCCESS : FAST_FSS_RUNTIME_ERROR;
 }
 
+// element sizes of the reference's API: the built-in widths and the 16-,
+// 24- and 32-byte wide integers (src/impl/def.h:37-44)
 inline bool elem_size_ok(size_t es)
+{
+    return es == 1 || es == 2 || es == 4 || es == 8 || es == 16 || es == 24 || es == 32;
+}
+
+// the widths of the fixed-width kernels (everything but wide.hip)
+inline bool elem_size_narrow(size_t es)
 {
     return es == 1 || es == 2 || es == 4 || es == 8;
 }

@@ -42,6 +42,37 @@ int launch_spline_eval_rt(size_t es, int grid, hipStream_t st, void *out, const 
                           const void *right, size_t I, int degree, int n, int m, size_t N, uint32_t *counter,
                           int block = kBlock);
 
+// Elements of 16, 24 and 32 bytes (wide.hip): one lane per element, for
+// arguments the entry point has already validated.  DPF mode 0 is eval, 1
+// evalAll, 2 evalMulti (point, P); Grotto op 0 is eval (mode 0 lt, 1 le, 2 eq),
+// 1 MIC and 2 interval LUT, whose `bits` is N * I ints of scratch.
+int wide_dcf_keygen(hipStream_t st, void *key, const void *alpha, const void *beta, const void *seed0,
+                    const void *seed1, int n, int m, size_t g, size_t es, size_t N);
+int wide_dcf_eval(hipStream_t st, void *out, const void *x, const void *key, const void *seed, int party, int n, int m,
+                  size_t g, size_t es, size_t N);
+int wide_dpf_keygen(hipStream_t st, void *key, const void *alpha, const void *beta, const void *seed0,
+                    const void *seed1, int n, int m, size_t g, size_t es, size_t N);
+int wide_dpf_eval(hipStream_t st, int mode, void *out, const void *x, const void *key, const void *seed, int party,
+                  int n, int m, size_t g, size_t es, size_t N, const void *point, size_t P);
+int wide_grotto_keygen(hipStream_t st, void *key, const void *alpha, const void *seed0, const void *seed1, int n,
+                       size_t es, size_t N);
+int wide_grotto_eval(hipStream_t st, int op, int mode, void *out, void *outT, void *bits, const void *x,
+                     const void *key, const void *seed, int party, int n, size_t es, size_t N, const void *left,
+                     const void *right, size_t I, const void *lut, size_t lutNum);
+int wide_mic_keygen(hipStream_t st, void *key, void *z, const void *alpha, const void *seed0, const void *seed1,
+                    const void *left, const void *right, size_t I, int n, int m, size_t es, size_t N);
+int wide_mic_eval(hipStream_t st, void *out, const void *x, const void *key, const void *z, const void *seed,
+                  int party, const void *left, const void *right, size_t I, int n, int m, size_t es, size_t N);
+int wide_spline_keygen(hipStream_t st, void *key, void *e, void *beta, const void *alpha, const void *seed0,
+                       const void *seed1, const void *coef, int d, const void *left, const void *right, size_t I,
+                       int n, int m, size_t es, size_t N);
+int wide_spline_eval(hipStream_t st, void *out, const void *x, const void *key, const void *e, const void *beta,
+                     const void *seed, int party, const void *left, const void *right, size_t I, int d, int n, int m,
+                     size_t es, size_t N, void *cache);
+int wide_ottt_keygen(hipStream_t st, void *key, const void *alpha, int n, size_t es, size_t N);
+int wide_ottt_eval(hipStream_t st, void *outE, void *outT, const void *x, const void *key, int party, const void *lut,
+                   size_t lutNum, int n, size_t es, size_t N);
+
 // Bytes of one entry of the device group table of a fused batch launch.
 size_t batch_table_entry_bytes();
 

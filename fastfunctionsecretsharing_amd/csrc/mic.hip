@@ -176,6 +176,13 @@ extern "C" FAST_FSS_API int FastFss_cuda_dcfMICKeyGen(void *key, size_t keyDataS
     if (N == 0) return FAST_FSS_SUCCESS;
     if (!aligned(key, 8) || !aligned(seed0, 8) || !aligned(seed1, 8)) return FAST_FSS_RUNTIME_ERROR;
     hipStream_t st = resolve_stream(cudaStreamPtr);
+    if (es > 8)
+    {
+        if (!aligned(z, 8) || !aligned(alpha, 8) || !aligned(leftEndpoints, 8) || !aligned(rightEndpoints, 8))
+            return FAST_FSS_RUNTIME_ERROR;
+        return wide_mic_keygen(st, key, z, alpha, seed0, seed1, leftEndpoints, rightEndpoints, I, (int)n, (int)m, es,
+                               N);
+    }
     uint32_t *counter = queue_counter_acquire(st);
     if (!counter) return FAST_FSS_RUNTIME_ERROR;
     FSS_DISPATCH(es, mic_keygen_kernel<T><<<FSS_SHAPE(N), 0, st>>>(
@@ -212,6 +219,14 @@ extern "C" FAST_FSS_API int FastFss_cuda_dcfMICEval(void *sharedOut, size_t shar
     if (N == 0 || I == 0) return FAST_FSS_SUCCESS;
     if (!aligned(key, 8) || !aligned(seed, 8)) return FAST_FSS_RUNTIME_ERROR;
     hipStream_t st = resolve_stream(cudaStreamPtr);
+    if (es > 8)
+    {
+        if (!aligned(sharedOut, 8) || !aligned(maskedX, 8) || !aligned(sharedZ, 8) || !aligned(leftEndpoints, 8) ||
+            !aligned(rightEndpoints, 8))
+            return FAST_FSS_RUNTIME_ERROR;
+        return wide_mic_eval(st, sharedOut, maskedX, key, sharedZ, seed, partyId, leftEndpoints, rightEndpoints, I,
+                             (int)n, (int)m, es, N);
+    }
     const size_t bpe     = mic_blocks_per_elem(N, I);
     const LaunchShape ls = shape_for(N * bpe);
     // measured on MI355X (16 intervals, n = 24): the static split is faster

@@ -8,6 +8,7 @@
 #include "FastFss/cuda/prng.h"
 #include "fss_device.h"
 #include "fss_host.h"
+#include "fss_launch.h"
 
 namespace fss {
 
@@ -504,6 +505,11 @@ extern "C" FAST_FSS_API int FastFss_cuda_otttKeyGen(void *key, size_t keyDataSiz
     if (alphaDataSize != N * es) return FAST_FSS_INVALID_ALPHA_DATA_SIZE_ERROR;
     if (N == 0) return FAST_FSS_SUCCESS;
     hipStream_t st = resolve_stream(cudaStreamPtr);
+    if (es > 8)
+    {
+        if (!aligned(alpha, 8)) return FAST_FSS_RUNTIME_ERROR;
+        return wide_ottt_keygen(st, key, alpha, (int)n, es, N);
+    }
     FSS_DISPATCH(es, ottt_keygen_kernel<T><<<FSS_SHAPE(N), 0, st>>>((uint8_t *)key, (const T *)alpha, (int)n,
                                                                             N));
     return launch_status();
@@ -532,6 +538,12 @@ extern "C" FAST_FSS_API int FastFss_cuda_otttLutEval(void *sharedOutE, size_t sh
     if (n >= 5 && !aligned(key, 4)) return FAST_FSS_RUNTIME_ERROR;
     if (n > 30) return FAST_FSS_RUNTIME_ERROR; // keys of 2^31 bits and more per element are not served
     hipStream_t st = resolve_stream(cudaStreamPtr);
+    if (es > 8)
+    {
+        if (!aligned(sharedOutE, 8) || !aligned(sharedOutT, 8) || !aligned(maskedX, 8) || !aligned(lookUpTable, 8))
+            return FAST_FSS_RUNTIME_ERROR;
+        return wide_ottt_eval(st, sharedOutE, sharedOutT, maskedX, key, partyId, lookUpTable, lutNum, (int)n, es, N);
+    }
     // passes of up to kOtttTables tables; each pass's tables sit in LDS, in chunks of C entries when
     // they do not fit in kOtttLdsBytes
     for (size_t t0 = 0; t0 < lutNum || t0 == 0; t0 += kOtttTables)

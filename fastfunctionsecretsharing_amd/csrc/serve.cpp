@@ -101,7 +101,7 @@ static size_t request_bytes(const FastFssServeRequest &r)
 static int validate(const FastFssServeRequest &r)
 {
     const size_t n = r.bitWidthIn, m = r.bitWidthOut, es = r.elementSize, N = r.elementNum;
-    if (!elem_size_ok(es)) return FAST_FSS_INVALID_ELEMENT_SIZE_ERROR;
+    if (!elem_size_narrow(es)) return FAST_FSS_INVALID_ELEMENT_SIZE_ERROR;
     if (!(0 < n && n <= es * 8)) return FAST_FSS_INVALID_BITWIDTH_ERROR;
     if (!(0 < m && m <= es * 8)) return FAST_FSS_INVALID_BITWIDTH_ERROR;
     if (r.partyId != 0 && r.partyId != 1) return FAST_FSS_INVALID_PARTY_ID_ERROR;

@@ -211,6 +211,14 @@ extern "C" FAST_FSS_API int FastFss_cuda_dcfSplineKeyGen(
     if (N == 0) return FAST_FSS_SUCCESS;
     if (!aligned(key, 8) || !aligned(seed0, 8) || !aligned(seed1, 8)) return FAST_FSS_RUNTIME_ERROR;
     hipStream_t st = resolve_stream(cudaStreamPtr);
+    if (es > 8)
+    {
+        if (!aligned(e, 8) || !aligned(beta, 8) || !aligned(alpha, 8) || !aligned(coefficients, 8) ||
+            !aligned(leftEndpoints, 8) || !aligned(rightEndpoints, 8))
+            return FAST_FSS_RUNTIME_ERROR;
+        return wide_spline_keygen(st, key, e, beta, alpha, seed0, seed1, coefficients, (int)degree, leftEndpoints,
+                                  rightEndpoints, I, (int)n, (int)m, es, N);
+    }
     const int ob = keygen_chunk_ob((int)m, es);
     FSS_DISPATCH(es, FSS_KEYGEN_OB_DISPATCH(
                          T, ob,
@@ -248,6 +256,14 @@ extern "C" FAST_FSS_API int FastFss_cuda_dcfSplineEval(
     if (N == 0) return FAST_FSS_SUCCESS;
     if (!aligned(key, 8) || !aligned(seed, 8)) return FAST_FSS_RUNTIME_ERROR;
     hipStream_t st = resolve_stream(cudaStreamPtr);
+    if (es > 8)
+    {
+        if (!aligned(sharedOut, 8) || !aligned(maskedX, 8) || !aligned(sharedE, 8) || !aligned(sharedBeta, 8) ||
+            !aligned(leftEndpoints, 8) || !aligned(rightEndpoints, 8) || !aligned(cache, 8))
+            return FAST_FSS_RUNTIME_ERROR;
+        return wide_spline_eval(st, sharedOut, maskedX, key, sharedE, sharedBeta, seed, partyId, leftEndpoints,
+                                rightEndpoints, I, (int)degree, (int)n, (int)m, es, N, cache);
+    }
     if (degree <= 7)
     {
         uint32_t *counter = queue_counter_acquire(st);
