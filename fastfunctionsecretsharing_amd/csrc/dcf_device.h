@@ -310,13 +310,14 @@ __device__ __forceinline__ void dcf_eval_range(const AesCtx &c, T acc[R], const 
 // dcf_eval_range for a compile-time output width OB = ceil(bitWidthOut/8):
 // the slots' conversion shares one key schedule per level across its AES
 // windows and extracts bytes at static positions (SlotConv).  Requires
-// cnt * OB <= 32 and cnt <= R.  Same results as dcf_eval_range.
-template <typename T, int OB, int R>
+// cnt <= R and cnt * OB <= 16 * (NW - 1) (a misaligned run of slots touches
+// one window more).  Same results as dcf_eval_range.
+template <typename T, int OB, int R, int NW = 3>
 __device__ __forceinline__ void dcf_eval_slots(const AesCtx &c, T acc[R], const uint8_t *sCW, const T *tail, T x,
                                                const uint32_t seed[4], int party, int n, int g, int j0, int cnt)
 {
     constexpr int NWORDS = (R * OB + 3) / 4;
-    SlotConv<OB, 3> cv;
+    SlotConv<OB, NW> cv;
     cv.setup(g, j0, cnt);
     uint32_t s[4] = {seed[0] & ~1u, seed[1], seed[2], seed[3]};
     uint32_t t    = (uint32_t)party;
@@ -346,7 +347,7 @@ __device__ __forceinline__ void dcf_eval_slots(const AesCtx &c, T acc[R], const 
         {
             if (r < cnt)
             {
-                T term = (T)((T)SlotConv<OB, 3>::template at<NWORDS>(words, r) + (t ? vcw[r] : (T)0));
+                T term = (T)((T)SlotConv<OB, NW>::template at<NWORDS>(words, r) + (t ? vcw[r] : (T)0));
                 acc[r] = (T)(acc[r] + neg_if<T>(term, party));
             }
         }
@@ -360,7 +361,7 @@ __device__ __forceinline__ void dcf_eval_slots(const AesCtx &c, T acc[R], const 
     {
         if (r < cnt)
         {
-            T term = (T)((T)SlotConv<OB, 3>::template at<NWORDS>(words, r) + (t ? last[r] : (T)0));
+            T term = (T)((T)SlotConv<OB, NW>::template at<NWORDS>(words, r) + (t ? last[r] : (T)0));
             acc[r] = (T)(acc[r] + neg_if<T>(term, party));
         }
     }

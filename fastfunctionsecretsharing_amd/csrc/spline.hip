@@ -117,7 +117,7 @@ uint32_t spline_blocks_per_elem(size_t N, size_t I)
     return b;
 }
 
-template <typename T, int OB>
+template <typename T, int OB, int BY = 32>
 static void spline_launch_ob(int grid, int block, hipStream_t st, void *out, const void *x, const void *key, const void *e,
                              const void *beta, const void *seed, int party, const void *left, const void *right,
                              size_t I, int degree, int n, int m, size_t N, uint32_t *counter)
@@ -127,18 +127,20 @@ static void spline_launch_ob(int grid, int block, hipStream_t st, void *out, con
     uint32_t bpe = spline_blocks_per_elem(N, I);
     while (bpe > 1 && N * bpe > (size_t)grid * (size_t)block) bpe >>= 1;
     if (bpe > 1)
-        spline_eval_split_kernel<T, OB><<<grid, block, 0, st>>>(
+        spline_eval_split_kernel<T, OB, BY><<<grid, block, 0, st>>>(
             (T *)out, (const T *)x, (const uint8_t *)key, (const T *)e, (const T *)beta, (const uint8_t *)seed, party,
             (const T *)left, (const T *)right, I, degree, n, m, N, bpe, counter);
     else
-        spline_eval_kernel<T, OB><<<grid, block, 0, st>>>((T *)out, (const T *)x, (const uint8_t *)key, (const T *)e,
+        spline_eval_kernel<T, OB, BY><<<grid, block, 0, st>>>((T *)out, (const T *)x, (const uint8_t *)key, (const T *)e,
                                                            (const T *)beta, (const uint8_t *)seed, party,
                                                            (const T *)left, (const T *)right, I, degree, n, m, N,
                                                            counter);
 }
 
 // degree <= 7: the slot-conversion kernel for OB = ceil(m/8) when a linked
-// walk's 2(degree+1) slots fit 32 bytes, else the generic conversion.
+// walk's 2(degree+1) slots fit 32 bytes, or 48 bytes for 8-byte slots
+// (degree <= 2 at bitWidthOut > 56), else the generic conversion, which pays
+// a key schedule per 16-byte window.
 template <typename T>
 static void spline_launch(int grid, int block, hipStream_t st, void *out, const void *x, const void *key, const void *e,
                           const void *beta, const void *seed, int party, const void *left, const void *right, size_t I,
@@ -147,6 +149,15 @@ static void spline_launch(int grid, int block, hipStream_t st, void *out, const 
     const int ob = (m + 7) / 8;
     // (64-bit elements with 2-byte outputs keep the generic kernel: its register footprint is smaller)
     const int fast = (2 * (degree + 1) * ob <= 32 && !(sizeof(T) == 8 && ob == 2)) ? ob : 0;
+    if constexpr (sizeof(T) == 8)
+    {
+        if (fast == 0 && ob == 8 && 2 * (degree + 1) * ob <= 48)
+        {
+            spline_launch_ob<T, 8, 48>(grid, block, st, out, x, key, e, beta, seed, party, left, right, I, degree, n,
+                                       m, N, counter);
+            return;
+        }
+    }
 #define FSS_SPLINE_OB(K)                                                                                            \
     case K:                                                                                                         \
         if constexpr (K <= (int)sizeof(T))                                                                          \

@@ -17,30 +17,31 @@ namespace fss {
 // once; the walk at x-1-(R_q+1) also yields interval q+1's "current" slots when
 // intervals q and q+1 are adjacent (the reference copies the whole vector,
 // src/impl/spline.h:214-223).  OB > 0 selects the slot conversion specialised
-// for ceil(bitWidthOut/8) = OB bytes (needs 2*(degree+1)*OB <= 32); OB = 0 the
-// generic one.  An element's intervals may be split into bpe blocks (a power
+// for ceil(bitWidthOut/8) = OB bytes (needs 2*(degree+1)*OB <= BY, the walk's
+// byte budget: 32, or 48 for 8-byte slots); OB = 0 the generic one.  An element's intervals may be split into bpe blocks (a power
 // of two dividing 64) on consecutive lanes: each lane sums its intervals'
 // terms of the polynomial, the first interval of a block walks its own start
 // point instead of taking it from its predecessor's walk (the same value), and
 // the bpe partial sums are added across the lanes, whose results are equal to
 // the one-lane walk's mod 2^m.
-template <typename T, int OB>
+template <typename T, int OB, int BY>
 __device__ __forceinline__ void spline_walk(const AesCtx &c, T *acc, const uint8_t *sc, const T *tail, T pt,
                                             const uint32_t s[4], int party, int n, int m, int g, int j0, int cnt)
 {
     if constexpr (OB == 0)
         dcf_eval_range<T, 16>(c, acc, sc, tail, pt, s, party, n, m, g, j0, cnt);
     else
-        dcf_eval_slots<T, OB, (32 / OB < 16 ? 32 / OB : 16)>(c, acc, sc, tail, pt, s, party, n, g, j0, cnt);
+        dcf_eval_slots<T, OB, (BY / OB < 16 ? BY / OB : 16), BY / 16 + 1>(c, acc, sc, tail, pt, s, party, n, g, j0,
+                                                                           cnt);
 }
 
-template <typename T, int OB, bool SPLIT>
+template <typename T, int OB, bool SPLIT, int BY>
 __device__ __forceinline__ void spline_eval_body(const AesCtx &c, T *out, const T *x, const uint8_t *key, const T *sE,
                                                  const T *sBeta, const uint8_t *seed, int party, const T *left,
                                                  const T *right, size_t I, int d, int n, int m, size_t N, uint32_t bpe,
                                                  uint32_t *counter)
 {
-    constexpr int RS = (OB == 0) ? 16 : (32 / OB < 16 ? 32 / OB : 16); // slots per walk
+    constexpr int RS = (OB == 0) ? 16 : (BY / OB < 16 ? BY / OB : 16); // slots per walk
     if constexpr (!SPLIT) bpe = 1;
     const int    C     = d + 1;
     const size_t g     = I * (size_t)C;
@@ -68,13 +69,13 @@ __device__ __forceinline__ void spline_eval_body(const AesCtx &c, T *out, const 
             T       acc[RS];
             if (!(q > q0 && mod_bits<T>((T)(right[q - 1] + 1), n) == L))
             {
-                spline_walk<T, OB>(c, acc, sc, tail, (T)(xv - 1 - L), s, party, n, m, (int)g, (int)(q * C), C);
+                spline_walk<T, OB, BY>(c, acc, sc, tail, (T)(xv - 1 - L), s, party, n, m, (int)g, (int)(q * C), C);
 #pragma unroll
                 for (int j = 0; j < 8; j++)
                     if (j < C) w[j] = (T)(w[j] - acc[j < RS ? j : 0]);
             }
             const bool link = (q + 1 < q1) && mod_bits<T>((T)(right[q] + 1), n) == mod_bits<T>(left[q + 1], n);
-            spline_walk<T, OB>(c, acc, sc, tail, (T)(xv - 1 - Rp), s, party, n, m, (int)g, (int)(q * C),
+            spline_walk<T, OB, BY>(c, acc, sc, tail, (T)(xv - 1 - Rp), s, party, n, m, (int)g, (int)(q * C),
                                link ? 2 * C : C);
             const T cx = (T)((int)(xv > L) - (int)(xv > Rp));
 #pragma unroll
@@ -119,19 +120,19 @@ __device__ __forceinline__ void spline_eval_body(const AesCtx &c, T *out, const 
     });
 }
 
-template <typename T, int OB>
+template <typename T, int OB, int BY = 32>
 __global__ __launch_bounds__(kBlock) void spline_eval_kernel(T *out, const T *x, const uint8_t *key, const T *sE,
                                                              const T *sBeta, const uint8_t *seed, int party,
                                                              const T *left, const T *right, size_t I, int d, int n,
                                                              int m, size_t N, uint32_t *counter)
 {
     FSS_KERNEL_PROLOGUE
-    spline_eval_body<T, OB, false>(c, out, x, key, sE, sBeta, seed, party, left, right, I, d, n, m, N, 1, counter);
+    spline_eval_body<T, OB, false, BY>(c, out, x, key, sE, sBeta, seed, party, left, right, I, d, n, m, N, 1, counter);
 }
 
 // The same with each element's intervals split into bpe > 1 blocks (calls
 // too small to give every CU a full workgroup).
-template <typename T, int OB>
+template <typename T, int OB, int BY = 32>
 __global__ __launch_bounds__(kBlock) void spline_eval_split_kernel(T *out, const T *x, const uint8_t *key,
                                                                    const T *sE, const T *sBeta, const uint8_t *seed,
                                                                    int party, const T *left, const T *right, size_t I,
@@ -139,7 +140,7 @@ __global__ __launch_bounds__(kBlock) void spline_eval_split_kernel(T *out, const
                                                                    uint32_t *counter)
 {
     FSS_KERNEL_PROLOGUE_STAGED
-    spline_eval_body<T, OB, true>(c, out, x, key, sE, sBeta, seed, party, left, right, I, d, n, m, N, bpe, counter);
+    spline_eval_body<T, OB, true, BY>(c, out, x, key, sE, sBeta, seed, party, left, right, I, d, n, m, N, bpe, counter);
 }
 
 } // namespace fss
