@@ -18,7 +18,8 @@ namespace fss {
 // intervals q and q+1 are adjacent (the reference copies the whole vector,
 // src/impl/spline.h:214-223).  OB > 0 selects the slot conversion specialised
 // for ceil(bitWidthOut/8) = OB bytes (needs 2*(degree+1)*OB <= BY, the walk's
-// byte budget: 32, or 48 for 8-byte slots); OB = 0 the generic one.  An element's intervals may be split into bpe blocks (a power
+// byte budget: 32, or 48 for 8-byte slots); OB = 0 the generic one.  An
+// element's intervals may be split into bpe blocks (a power
 // of two dividing 64) on consecutive lanes: each lane sums its intervals'
 // terms of the polynomial, the first interval of a block walks its own start
 // point instead of taking it from its predecessor's walk (the same value), and
