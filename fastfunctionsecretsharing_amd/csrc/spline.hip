@@ -105,16 +105,23 @@ __global__ __launch_bounds__(kBlock) void spline_eval_wide_kernel(T *out, const 
     }
 }
 
+// The smallest split of an element's intervals: 2 from 16 intervals, else 1.
+// With every lane busy, the 2-way split of 16 adjacent intervals costs 18
+// lane-walks per element instead of 17 (9 steps on two lanes) and still ran
+// 2.5-5% faster at 2^20 and 2^22 elements in same-process A/B runs; with 4
+// intervals (6 lane-walks instead of 5) it ran 7-10% slower.
+static uint32_t spline_min_split(size_t I) { return I >= 16 ? 2u : 1u; }
+
 // Interval blocks per element for the spline walk: a power of two dividing
 // 64 (so an element's lanes share one wave), at most I, and enough lanes to
-// give every CU a full workgroup.
+// give every CU a full workgroup, or spline_min_split(I) if that is more.
 uint32_t spline_blocks_per_elem(size_t N, size_t I)
 {
     const size_t lanes = (size_t)device_cu_count() * kBlock;
     const size_t want  = (lanes + N - 1) / N;
     uint32_t     b     = 1;
     while (b * 2 <= 64 && b * 2 <= want && b * 2 <= I) b *= 2;
-    return b;
+    return b < spline_min_split(I) ? spline_min_split(I) : b;
 }
 
 template <typename T, int OB, int BY = 32>
@@ -123,9 +130,9 @@ static void spline_launch_ob(int grid, int block, hipStream_t st, void *out, con
                              size_t I, int degree, int n, int m, size_t N, uint32_t *counter)
 {
     // split only as far as the launch has lanes for (the serving runtime sizes
-    // its grid by elements; a split it has no lanes for only adds walks)
+    // its grid by elements; a wider split it has no lanes for only adds walks)
     uint32_t bpe = spline_blocks_per_elem(N, I);
-    while (bpe > 1 && N * bpe > (size_t)grid * (size_t)block) bpe >>= 1;
+    while (bpe > spline_min_split(I) && N * bpe > (size_t)grid * (size_t)block) bpe >>= 1;
     if (bpe > 1)
         spline_eval_split_kernel<T, OB, BY><<<grid, block, 0, st>>>(
             (T *)out, (const T *)x, (const uint8_t *)key, (const T *)e, (const T *)beta, (const uint8_t *)seed, party,

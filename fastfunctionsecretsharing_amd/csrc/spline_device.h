@@ -19,12 +19,14 @@ namespace fss {
 // src/impl/spline.h:214-223).  OB > 0 selects the slot conversion specialised
 // for ceil(bitWidthOut/8) = OB bytes (needs 2*(degree+1)*OB <= BY, the walk's
 // byte budget: 32, or 48 for 8-byte slots); OB = 0 the generic one.  An
-// element's intervals may be split into bpe blocks (a power
-// of two dividing 64) on consecutive lanes: each lane sums its intervals'
-// terms of the polynomial, the first interval of a block walks its own start
-// point instead of taking it from its predecessor's walk (the same value), and
-// the bpe partial sums are added across the lanes, whose results are equal to
-// the one-lane walk's mod 2^m.
+// element's intervals may be split into bpe blocks (a power of two dividing
+// 64) on consecutive lanes of one wave, which then read the element's seed
+// corrections together: each lane sums its terms of the polynomial and the
+// bpe partial sums are added across the lanes.  The linked walk that ends a
+// block also subtracts the next block's first "current" slots, so the split
+// walks the same points as one lane (I + 1 for I adjacent intervals), and
+// since every term enters the sum exactly once the result equals the one-lane
+// walk's mod 2^m.
 template <typename T, int OB, int BY>
 __device__ __forceinline__ void spline_walk(const AesCtx &c, T *acc, const uint8_t *sc, const T *tail, T pt,
                                             const uint32_t s[4], int party, int n, int m, int g, int j0, int cnt)
@@ -68,14 +70,16 @@ __device__ __forceinline__ void spline_eval_body(const AesCtx &c, T *out, const 
             const T L  = mod_bits<T>(left[q], n), R = mod_bits<T>(right[q], n);
             const T Rp = mod_bits<T>((T)(R + 1), n);
             T       acc[RS];
-            if (!(q > q0 && mod_bits<T>((T)(right[q - 1] + 1), n) == L))
+            // interval q's start slots come from the linked end walk of q - 1
+            // (on this lane or, at a block boundary, on the previous one)
+            if (!(q > 0 && mod_bits<T>((T)(right[q - 1] + 1), n) == L))
             {
                 spline_walk<T, OB, BY>(c, acc, sc, tail, (T)(xv - 1 - L), s, party, n, m, (int)g, (int)(q * C), C);
 #pragma unroll
                 for (int j = 0; j < 8; j++)
                     if (j < C) w[j] = (T)(w[j] - acc[j < RS ? j : 0]);
             }
-            const bool link = (q + 1 < q1) && mod_bits<T>((T)(right[q] + 1), n) == mod_bits<T>(left[q + 1], n);
+            const bool link = (q + 1 < I) && mod_bits<T>((T)(right[q] + 1), n) == mod_bits<T>(left[q + 1], n);
             spline_walk<T, OB, BY>(c, acc, sc, tail, (T)(xv - 1 - Rp), s, party, n, m, (int)g, (int)(q * C),
                                link ? 2 * C : C);
             const T cx = (T)((int)(xv > L) - (int)(xv > Rp));
@@ -131,8 +135,9 @@ __global__ __launch_bounds__(kBlock) void spline_eval_kernel(T *out, const T *x,
     spline_eval_body<T, OB, false, BY>(c, out, x, key, sE, sBeta, seed, party, left, right, I, d, n, m, N, 1, counter);
 }
 
-// The same with each element's intervals split into bpe > 1 blocks (calls
-// too small to give every CU a full workgroup).
+// The same with each element's intervals split into bpe > 1 blocks: calls too
+// small to give every CU a full workgroup, and calls of many intervals, whose
+// 2-way split pays for its extra wave step even when every lane is busy.
 template <typename T, int OB, int BY = 32>
 __global__ __launch_bounds__(kBlock) void spline_eval_split_kernel(T *out, const T *x, const uint8_t *key,
                                                                    const T *sE, const T *sBeta, const uint8_t *seed,
