@@ -409,7 +409,8 @@ def dpf_leg(log2n: int, reps: int = 10):
 def small_calls_leg(sizes=(1024, 16384), reps=50):
     """Per-call kernel time of small calls, which fill only part of the GPU: DCF and DPF eval at
     bitWidthIn = bitWidthOut = 32, and the degree-2 spline of 16 intervals at bitWidthIn = 24,
-    bitWidthOut = 32; one party, HIP events over `reps` back-to-back calls on one stream."""
+    bitWidthOut = 32; one party, HIP events over `reps` back-to-back calls on one stream, right after
+    `reps` untimed calls."""
     import torch
 
     from fastfunctionsecretsharing_amd import _lib, workloads as W
@@ -431,14 +432,16 @@ def small_calls_leg(sizes=(1024, 16384), reps=50):
                         g.seeds[0].data_ptr(), 16 * N, 0, 32, 32, 1, 4, N, None, 0, sp)
                 assert rc == 0, rc
 
-            call()
+            for _ in range(reps):  # warm-up calls right before the timed ones
+                call()
             _, ms = _timed(stream, call, reps)
             res["%s_n32_%d" % (name, N)] = {"elements": N, "kernel_ms_per_call": ms}
             del g, out
         w = W.piecewise_workload(N=N, mic=False)
         cache = W.spline_cache(w)
         out = torch.empty(N, dtype=torch.int32, device=dev)
-        W.spline_eval(w, 0, out, w.e, w.beta, cache, sp)
+        for _ in range(reps):
+            W.spline_eval(w, 0, out, w.e, w.beta, cache, sp)
         _, ms = _timed(stream, lambda: W.spline_eval(w, 0, out, w.e, w.beta, cache, sp), reps)
         res["spline_n24_%d" % N] = {"elements": N, "kernel_ms_per_call": ms}
         del w, cache, out
@@ -616,8 +619,10 @@ def piecewise_leg(reps: int = 3):
     full = {"workload": "full-domain Grotto [x < alpha] shares, bitWidthIn=16, 4096 keys x 2^16 points, party 0"}
     for name, packed, out in (("elements", 0, torch.empty(pts, dtype=torch.int16, device="cuda:0")),
                               ("packed_bits", 1, torch.empty(pts // 64, dtype=torch.int64, device="cuda:0"))):
-        W.grotto_full(gs, 0, _lib.GROTTO_FULL_LT, packed, out, sp)
-        wall, dev_ms = _timed(stream, lambda: W.grotto_full(gs, 0, _lib.GROTTO_FULL_LT, packed, out, sp), reps * 3)
+        # a launch takes about 0.1 ms: 100 warm-up calls right before 100 timed ones
+        for _ in range(100):
+            W.grotto_full(gs, 0, _lib.GROTTO_FULL_LT, packed, out, sp)
+        wall, dev_ms = _timed(stream, lambda: W.grotto_full(gs, 0, _lib.GROTTO_FULL_LT, packed, out, sp), 100)
         full[name] = {"points_per_s": pts / wall, "kernel_ms": dev_ms,
                       "lds_frac": lds_frac(lookups / (dev_ms * 1e-3)),
                       "out_gbs": out.numel() * out.element_size() / (dev_ms * 1e-3) / 1e9}

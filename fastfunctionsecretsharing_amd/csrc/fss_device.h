@@ -692,7 +692,8 @@ __device__ __forceinline__ T mod_bits(T x, int bw)
     __syncthreads();                             \
     const AesCtx c = aes_ctx(lds);
 
-// the same for kernels launched with workgroups below 1024 lanes (aes_fill_lds_staged)
+// the same for kernels launched with workgroups below 1024 lanes, or whose
+// whole launch lasts about 0.1 ms (aes_fill_lds_staged)
 #define FSS_KERNEL_PROLOGUE_STAGED                         \
     __shared__ __align__(16) uint32_t lds[kLdsTableWords]; \
     aes_fill_lds_staged(lds);                              \
