@@ -108,6 +108,8 @@ struct FastFssKeyStore
 
     int add_slot(size_t keyBytes, uint64_t *handle, size_t *offset)
     {
+        // checked before rounding up, which would wrap for sizes near SIZE_MAX
+        if (keyBytes > capacity) return FAST_FSS_RUNTIME_ERROR;
         const size_t bytes = round_up(keyBytes);
         const size_t off   = take(bytes);
         if (off == SIZE_MAX) return FAST_FSS_RUNTIME_ERROR;
@@ -122,8 +124,9 @@ struct FastFssKeyStore
 
 extern "C" FAST_FSS_API int FastFss_hip_keyStoreCreate(FastFssKeyStore **store, size_t capacityBytes)
 {
-    if (!store || capacityBytes == 0) return FAST_FSS_RUNTIME_ERROR;
+    if (!store) return FAST_FSS_RUNTIME_ERROR;
     *store = nullptr;
+    if (capacityBytes == 0 || capacityBytes > SIZE_MAX - kAlign) return FAST_FSS_RUNTIME_ERROR;
     auto  *s   = new FastFssKeyStore();
     size_t cap = round_up(capacityBytes);
     if (hipGetDevice(&s->device) != hipSuccess || hipMalloc((void **)&s->base, cap) != hipSuccess)

@@ -21,7 +21,8 @@
  *  - Every call is thread-safe.  Handles are never reused; 0 is no handle.
  *
  * Errors: FAST_FSS_SUCCESS, FAST_FSS_RUNTIME_ERROR for a null store or
- * pointer, an unknown handle, a zero size, no room, or a failed HIP call.
+ * pointer, an unknown handle, a zero size or one above the store's capacity,
+ * no room, or a failed HIP call.
  */
 #ifndef FAST_FSS_HIP_KEY_STORE_H
 #define FAST_FSS_HIP_KEY_STORE_H
@@ -46,7 +47,8 @@ FAST_FSS_API int FastFss_hip_keyStoreDestroy(FastFssKeyStore *store);
 
 /* A new slot holding a copy of keyDataSize bytes at `key` (host or device
  * memory), copied on *hipStreamPtr (NULL: the per-thread default stream);
- * the copy is ordered before later work on that stream. */
+ * the copy is ordered before later work on that stream, and the caller keeps
+ * `key` unchanged until it has run. */
 FAST_FSS_API int FastFss_hip_keyStorePut(FastFssKeyStore *store, const void *key, size_t keyDataSize,
                                          uint64_t *handle, void *hipStreamPtr);
 

@@ -202,11 +202,18 @@ def test_invalid_calls(dev):
         buf = np.zeros(16, np.uint8)
         assert st.put(buf.ctypes.data, 0)[0] != 0
         assert st.put(buf.ctypes.data, 8192)[0] != 0  # larger than the store
+        # sizes whose round-up to 256 bytes would wrap around: rejected before any copy
+        assert st.put(buf.ctypes.data, 2**64 - 1)[0] != 0
+        assert st.put(buf.ctypes.data, 2**64 - 200)[0] != 0
+        assert st.reserve(2**64 - 1)[0] != 0
         assert st.get(12345)[0] != 0
         assert st.release(12345) != 0
         assert st.reserve(0)[0] != 0
+        assert st.stats()[0] == 0 and st.stats()[2] == 0  # nothing was handed out
     finally:
         st.destroy()
+    assert lib.FastFss_hip_keyStoreCreate(ctypes.addressof(h), 2**64 - 1) != 0
+    assert lib.FastFss_hip_keyStoreCreate(ctypes.addressof(h), 2**64 - 100) != 0
 
 
 def test_concurrent_threads(dev):
