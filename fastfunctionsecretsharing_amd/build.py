@@ -44,13 +44,20 @@ CFLAGS = [
 ]
 
 
-# Per-source flags.  spline_eval.hip: LLVM's iterative ILP scheduler keeps up to
-# 16 AES table lookups in flight per wave in the spline walk instead of 1-3
-# (7% faster at 2^22 elements, 35% at 1024); it crashes the compile of
-# spline.hip's keyGen kernels, which is why the eval launchers have a file of
-# their own.
+# Per-source flags: LLVM's iterative ILP scheduler, where same-process A/B runs
+# showed it faster with equal outputs.
+#  - spline_eval.hip: a wave keeps up to 16 AES table lookups in flight in the
+#    spline walk instead of 1-3 (7% faster at 2^22 elements, 35% at 1024).  The
+#    option crashes the compile of spline.hip's keyGen kernels (and of dcf.hip
+#    and mic.hip), which is why the eval launchers have a file of their own.
+#  - dpf_grotto.hip: DPF eval of 2^14 elements 10% faster, 2^24 and evalAll
+#    0.3-1%, full-domain Grotto unchanged.
+#  Measured and left on the default scheduler: ottt_prng.hip (OTTT 16-26%
+#  slower) and batch.hip (ragged batch unchanged).
+ILP_SCHEDULER = ["-mllvm", "-amdgpu-sched-strategy=iterative-ilp"]
 SOURCE_FLAGS = {
-    "spline_eval.hip": ["-mllvm", "-amdgpu-sched-strategy=iterative-ilp"],
+    "spline_eval.hip": ILP_SCHEDULER,
+    "dpf_grotto.hip": ILP_SCHEDULER,
 }
 
 
