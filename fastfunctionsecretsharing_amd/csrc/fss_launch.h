@@ -27,6 +27,13 @@ int launch_dcf_eval_g1_rt(size_t es, int grid, hipStream_t st, void *out, const 
 int launch_dpf_eval_g1_rt(size_t es, int grid, hipStream_t st, void *out, const void *x, const void *key,
                           const void *seed, int party, int n, int m, size_t N, int block = kBlock);
 
+// MIC calls of at least this many elements run on the work queue, smaller
+// ones on the static grid split.  Measured on MI355X (16 intervals, same
+// process): the queue is 10% faster at 2^19-2^20 elements for n = 64 (2^20:
+// 23.9 against 26.7 ms) and 0-2% for n = 16 and 32; the static split is 1-12%
+// faster at 2^16-2^18.
+constexpr size_t kMicQueueMinElems = (size_t)1 << 19;
+
 // counter: zeroed work-queue counter, or nullptr for the static grid split
 int launch_mic_eval_rt(size_t es, int grid, hipStream_t st, void *out, const void *x, const void *key, const void *z,
                        const void *seed, int party, const void *left, const void *right, size_t I, int n, int m,

@@ -229,10 +229,9 @@ extern "C" FAST_FSS_API int FastFss_cuda_dcfMICEval(void *sharedOut, size_t shar
     }
     const size_t bpe     = mic_blocks_per_elem(N, I);
     const LaunchShape ls = shape_for(N * bpe);
-    // measured on MI355X (16 intervals, n = 24): the static split is faster
-    // up to 2^20 elements (2^14: 282 against 336 us), the work queue from
-    // 2^22 (32.3 against 36.7 ms)
-    if (N < ((size_t)1 << 21))
+    // the static split below kMicQueueMinElems (2^14 elements, n = 24: 282
+    // against 336 us), the work queue from there (2^22: 32.3 against 36.7 ms)
+    if (N < kMicQueueMinElems)
         return launch_mic_eval_rt(es, ls.grid, st, sharedOut, maskedX, key, sharedZ, seed, partyId, leftEndpoints,
                                   rightEndpoints, I, (int)n, (int)m, N, nullptr, ls.block);
     uint32_t *counter = queue_counter_acquire(st);

@@ -151,15 +151,26 @@ static int launch(Server *s, ServeItem *it, hipStream_t st)
         case FAST_FSS_SERVE_DPF_EVAL:
             return launch_dpf_eval_g1_rt(r.elementSize, grid, st, r.sharedOut, r.maskedX, r.key, r.seed, r.partyId, n,
                                          m, r.elementNum, ls.block);
-        case FAST_FSS_SERVE_MIC_EVAL:
-            return launch_mic_eval_rt(r.elementSize, grid, st, r.sharedOut, r.maskedX, r.key, r.sharedZ, r.seed,
-                                      r.partyId, r.leftEndpoints, r.rightEndpoints, r.intervalNum, n, m,
-                                      r.elementNum, nullptr, ls.block);
-        default:
-            return launch_spline_eval_rt(r.elementSize, grid, st, r.sharedOut, r.maskedX, r.key, r.sharedZ,
-                                         r.sharedBeta, r.seed, r.partyId, r.leftEndpoints, r.rightEndpoints,
-                                         r.intervalNum, (int)r.degree, n, m, r.elementNum, nullptr, ls.block);
+        default: break;
     }
+    // MIC and spline: mode 0 takes the per-call entry points' work-queue
+    // choice (MIC from kMicQueueMinElems elements, the spline at every size);
+    // without a counter (mode 1, or none to be had) the same kernels run on
+    // the static grid split, which gives the same outputs
+    const bool wantQueue = s->launchMode == 0 &&
+                           (r.op == FAST_FSS_SERVE_SPLINE_EVAL || r.elementNum >= kMicQueueMinElems);
+    uint32_t  *counter   = wantQueue ? queue_counter_acquire(st) : nullptr;
+    int        rc;
+    if (r.op == FAST_FSS_SERVE_MIC_EVAL)
+        rc = launch_mic_eval_rt(r.elementSize, grid, st, r.sharedOut, r.maskedX, r.key, r.sharedZ, r.seed, r.partyId,
+                                r.leftEndpoints, r.rightEndpoints, r.intervalNum, n, m, r.elementNum, counter,
+                                ls.block);
+    else
+        rc = launch_spline_eval_rt(r.elementSize, grid, st, r.sharedOut, r.maskedX, r.key, r.sharedZ, r.sharedBeta,
+                                   r.seed, r.partyId, r.leftEndpoints, r.rightEndpoints, r.intervalNum, (int)r.degree,
+                                   n, m, r.elementNum, counter, ls.block);
+    if (counter) queue_counter_release(st, counter);
+    return rc;
 }
 
 static int pick_stream(Server *s)
