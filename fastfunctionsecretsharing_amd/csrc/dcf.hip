@@ -103,6 +103,29 @@ __global__ __launch_bounds__(kBlock) void dcf_eval_pair_kernel(T *out, const T *
     }
 }
 
+// Eight lanes per element (dcf_eval_g1_oct); the grid-stride step is a
+// multiple of 8, so a group of eight lanes always holds one element.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void dcf_eval_oct_kernel(T *out, const T *x, const uint8_t *key,
+                                                              const uint8_t *seed, int party, int n, int outBytes,
+                                                              size_t N)
+{
+    FSS_KERNEL_PROLOGUE_STAGED
+    const uint32_t q = threadIdx.x & 3u, h = (threadIdx.x >> 2) & 1u;
+    QuadAes        qa;
+    qa.init(c, q);
+    const size_t sw    = 16 * (size_t)n;
+    const T     *tails = (const T *)(key + N * sw);
+    const size_t ts    = (size_t)n + 3;
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; (g >> 3) < N; g += (size_t)gridDim.x * blockDim.x)
+    {
+        const size_t   e = g >> 3;
+        const uint32_t s = *(const uint32_t *)(seed + 16 * e + 4 * q);
+        const T        v = dcf_eval_g1_oct<T>(c, qa, key + e * sw, tails + e * ts, x[e], s, party, n, outBytes, h, q);
+        if ((threadIdx.x & 7u) == 4u) out[e] = v;
+    }
+}
+
 // E elements per lane: lane `gid` takes elements gid + k*S (k < E, S = total
 // lanes) per step of a grid-stride loop over E*S; surplus slots redo the
 // first element and are not stored.
@@ -265,13 +288,14 @@ void queue_counter_release(hipStream_t st, uint32_t *p)
 // grid-stride split, 5 = CH 8 bursts over the work queue, 6 = variant 5 with
 // each element's value corrections loaded up front (dcf_eval_g1_pre; elements
 // of at most 32 bits, 64-bit elements run variant 5), 7 = two lanes per
-// element (dcf_eval_g1_pair; its own launch shape over 2N lanes).
+// element (dcf_eval_g1_pair; its own launch shape over 2N lanes), 8 = eight
+// lanes per element (dcf_eval_g1_oct, column-parallel AES; over 8N lanes).
 struct EvalVariant
 {
     int ch, e, queue;
 };
 constexpr EvalVariant kEvalVariants[] = {{1, 1, 0}, {4, 1, 0}, {8, 1, 0}, {4, 2, 0},
-                                         {2, 2, 0}, {8, 1, 1}, {8, 1, 2}, {1, 1, 0}};
+                                         {2, 2, 0}, {8, 1, 1}, {8, 1, 2}, {1, 1, 0}, {1, 1, 0}};
 constexpr int         kNumEvalVariants = sizeof(kEvalVariants) / sizeof(kEvalVariants[0]);
 constexpr size_t      kQueueMinElems   = (size_t)1 << 20; // below this the static split is used
 
@@ -293,9 +317,13 @@ int dcf_eval_variant(size_t es, size_t n, size_t N)
     // (n = 32, 2^24: 11.3 instead of 19.0 GB fetched); below that its counter
     // setup and per-tile atomics cost more than they balance.  Lane pairs win
     // while they leave at most two waves per SIMD (2^16 elements: 72 against
-    // 88 us for the static split at n = 32).
+    // 88 us for the static split at n = 32), and eight lanes per element
+    // (column-parallel AES) while those do (n = 32: 1024 elements 36 against
+    // 49 us for lane pairs, 2^14 41 against 52; 2^15 64 against 54).
     if (N >= kQueueMinElems) return 6;
-    return 2 * N <= (size_t)device_cu_count() * 512 ? 7 : 2;
+    const size_t lanes = (size_t)device_cu_count() * 512;
+    if (8 * N <= lanes) return 8;
+    return 2 * N <= lanes ? 7 : 2;
 }
 
 // `fixed_counter`: a caller-owned tile counter for the work-queue variants
@@ -335,6 +363,12 @@ int launch_dcf_eval_g1(int v, int grid, int block, hipStream_t st, T *out, const
         dcf_eval_pair_kernel<T><<<ps.grid, ps.block, 0, st>>>(out, x, key, seed, party, n, outBytes, N);
         return FAST_FSS_SUCCESS;
     }
+    if (v == 8)
+    {
+        const LaunchShape ps = shape_for(8 * N);
+        dcf_eval_oct_kernel<T><<<ps.grid, ps.block, 0, st>>>(out, x, key, seed, party, n, outBytes, N);
+        return FAST_FSS_SUCCESS;
+    }
     switch (v)
     {
         case 0: dcf_eval_g1_kernel<T><<<grid, block, 0, st>>>(out, x, key, seed, party, n, outBytes, N); break;
@@ -347,7 +381,7 @@ int launch_dcf_eval_g1(int v, int grid, int block, hipStream_t st, T *out, const
 }
 
 // Serving launches choose their own grid shape, so they keep the static split
-// (variants 5, 6 and 7 fall back to variant 2 there).
+// (variants 5 to 8 fall back to variant 2 there).
 int launch_dcf_eval_g1_rt(size_t es, int grid, hipStream_t st, void *out, const void *x, const void *key,
                           const void *seed, int party, int n, int m, size_t N, int block)
 {

@@ -109,6 +109,59 @@ __device__ __forceinline__ T dcf_eval_g1_pair(const AesCtx &c, const uint8_t *sC
     return (T)(acc + neg_if<T>(term, party));
 }
 
+// dcf_eval_g1 on eight lanes per element, for calls too small to fill the
+// GPU, where one wave's instruction issue sets the time.  Lanes 0-3 of the
+// group (h = 0) encrypt the taken child's seed block (4x, 4x+1) and lanes 4-7
+// (h = 1) its value block (4x+2, 4x+3), one column per lane (QuadAes); the
+// seed block reaches the value lanes by a DPP row shift by 4.  Every lane
+// carries its column of the seed and the t bit down the tree; lane 4 (h = 1,
+// column 0) returns the share.  Same results as dcf_eval_g1.
+template <typename T>
+__device__ __forceinline__ T dcf_eval_g1_oct(const AesCtx &c, const QuadAes &qa, const uint8_t *sCW, const T *tail,
+                                             T x, uint32_t s, int party, int n, int outBytes, uint32_t h, uint32_t q)
+{
+    if (q == 0) s &= ~1u;
+    uint32_t t   = (uint32_t)party;
+    x            = mod_bits<T>(x, n);
+    T        acc = 0;
+    const T  tl = tail[n], tr = tail[n + 1];
+    // correction words one level ahead: this lane's column of level i's sCW
+    uint32_t cwNext = *(const uint32_t *)(sCW + 4 * q);
+    T        vNext  = tail[0];
+    for (int i = 0; i < n; i++)
+    {
+        const uint32_t cw  = cwNext;
+        const T        vcw = vNext;
+        if (i + 1 < n)
+        {
+            cwNext = *(const uint32_t *)(sCW + 16 * (i + 1) + 4 * q);
+            vNext  = tail[i + 1];
+        }
+        const uint32_t xi = (uint32_t)(x >> (n - 1 - i)) & 1u;
+        // plaintext columns (4x + 2h, 0, 4x + 2h + 1, 0)
+        const uint32_t pt  = (q & 1u) ? 0u : 4u * xi + 2u * h + (q >> 1);
+        const uint32_t blk = qa.encrypt(c, s, pt);
+        // the seed block's column q on both halves: lanes 4-7 read lanes 0-3
+        const uint32_t fromSeed = (uint32_t)__builtin_amdgcn_update_dpp((int)blk, (int)blk, 0x114, 0xF, 0xF, false);
+        const uint32_t sc       = h ? fromSeed : blk;
+        const uint32_t c0       = quad_mov<FSS_QP(0, 0, 0, 0)>(sc);
+        const uint32_t m        = 0u - t;
+        uint32_t       tn       = c0 & 1u;
+        s                       = ((q == 0) ? (sc & ~1u) : sc) ^ (cw & m);
+        tn ^= (uint32_t)(((xi ? tr : tl) >> i) & 1) & t;
+        // value block columns 0 and 1 (lane 4 and its right neighbour)
+        const uint32_t v1   = quad_mov<FSS_QP(1, 1, 1, 1)>(blk);
+        const uint32_t vb[4] = {blk, v1, 0u, 0u};
+        T term = (T)((T)low_bytes(vb, outBytes) + (t ? vcw : (T)0));
+        acc    = (T)(acc + neg_if<T>(term, party));
+        t      = tn;
+    }
+    const uint32_t s1    = quad_mov<FSS_QP(1, 1, 1, 1)>(s);
+    const uint32_t sb[4] = {s, s1, 0u, 0u};
+    T term = (T)((T)low_bytes(sb, outBytes) + (t ? tail[n + 2] : (T)0));
+    return (T)(acc + neg_if<T>(term, party));
+}
+
 // Groupsize-1 shares of E elements per lane with the correction words read in
 // bursts of CH levels.  A lane's sCW run is 16n contiguous bytes and its vCW
 // run n contiguous group elements; fetching CH levels back to back consumes a

@@ -189,6 +189,40 @@ __device__ __forceinline__ T dpf_eval_g1_burst(const AesCtx &c, const uint8_t *s
     return party ? (T)(0 - term) : term;
 }
 
+// dpf_eval_g1_burst on a quad of lanes per element, one column of the seed
+// block per lane (QuadAes), for calls too small to fill the GPU.  Lane 0 of
+// the quad returns the share.  Same results as dpf_eval_g1_burst.
+template <typename T>
+__device__ __forceinline__ T dpf_eval_g1_quad(const AesCtx &c, const QuadAes &qa, const uint8_t *sCW, const T *tail,
+                                              T xin, uint32_t s, int party, int n, int m, uint32_t q)
+{
+    if (q == 0) s &= ~1u;
+    uint32_t t  = (uint32_t)party;
+    const T  x  = mod_bits<T>(xin, n);
+    const T  tl = tail[0], tr = tail[1];
+    // this lane's column of the level's correction word, one level ahead
+    uint32_t cwNext = *(const uint32_t *)(sCW + 4 * q);
+    for (int i = 0; i < n; i++)
+    {
+        const uint32_t cw = cwNext;
+        if (i + 1 < n) cwNext = *(const uint32_t *)(sCW + 16 * (i + 1) + 4 * q);
+        const uint32_t bit = (uint32_t)(x >> (n - 1 - i)) & 1u;
+        // plaintext columns (2 bit, 0, 2 bit + 1, 0)
+        const uint32_t pt  = (q & 1u) ? 0u : 2u * bit + (q >> 1);
+        const uint32_t blk = qa.encrypt(c, s, pt);
+        const uint32_t c0  = quad_mov<FSS_QP(0, 0, 0, 0)>(blk);
+        const uint32_t mk  = 0u - t;
+        uint32_t       tn  = c0 & 1u;
+        s                  = ((q == 0) ? (blk & ~1u) : blk) ^ (cw & mk);
+        tn ^= (uint32_t)(((bit ? tr : tl) >> i) & 1) & t;
+        t = tn;
+    }
+    // byte 0 of the final seed, or of AES_seed(0^128) when bitWidthOut > 16
+    const uint32_t conv = (m <= 16 ? s : qa.encrypt(c, s, 0u)) & 0xffu;
+    const T        term = (T)((T)conv + (t ? tail[2] : (T)0));
+    return party ? (T)(0 - term) : term;
+}
+
 // ---- Grotto ----
 // key: [sCW N x (n-6) x 16][lastCW N x 8][tLCW, tRCW N x 2 GE] (src/impl/grotto.h:53-79)
 

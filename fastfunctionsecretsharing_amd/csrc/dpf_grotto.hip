@@ -62,6 +62,26 @@ __global__ __launch_bounds__(kBlock) void dpf_eval_burst_kernel(T *out, const T 
     }
 }
 
+// Four lanes per element (dpf_eval_g1_quad); the grid-stride step is a
+// multiple of 4, so a quad always holds one element.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void dpf_eval_quad_kernel(T *out, const T *x, const uint8_t *key,
+                                                               const uint8_t *seed, int party, int n, int m, size_t N)
+{
+    FSS_KERNEL_PROLOGUE_STAGED
+    const uint32_t q = threadIdx.x & 3u;
+    QuadAes        qa;
+    qa.init(c, q);
+    const T *tails = (const T *)(key + N * 16 * (size_t)n);
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; (g >> 2) < N; g += (size_t)gridDim.x * blockDim.x)
+    {
+        const size_t   e = g >> 2;
+        const uint32_t s = *(const uint32_t *)(seed + 16 * e + 4 * q);
+        const T        v = dpf_eval_g1_quad<T>(c, qa, key + e * 16 * (size_t)n, tails + e * 3, x[e], s, party, n, m, q);
+        if (q == 0) out[e] = v;
+    }
+}
+
 // Work-queue form (see dcf_eval_queue_kernel): 64-element tiles per wave.
 template <typename T, int CH>
 __device__ __forceinline__ void dpf_eval_queue_body(const AesCtx &c, T *out, const T *x, const uint8_t *key,
@@ -684,6 +704,12 @@ static int dpf_eval_head(int party, size_t n, size_t m, size_t g, size_t es)
     return FAST_FSS_SUCCESS;
 }
 
+// Four lanes per element while they leave at most two waves per SIMD.
+static bool dpf_quad_lanes(size_t N)
+{
+    return 4 * N <= (size_t)device_cu_count() * 512;
+}
+
 extern "C" FAST_FSS_API int FastFss_cuda_dpfEval(void *sharedOut, size_t sharedOutDataSize, const void *maskedX,
                                                  size_t maskedXDataSize, const void *key, size_t keyDataSize,
                                                  const void *seed, size_t seedDataSize, int partyId,
@@ -719,6 +745,12 @@ extern "C" FAST_FSS_API int FastFss_cuda_dpfEval(void *sharedOut, size_t sharedO
                              (T *)sharedOut, (const T *)maskedX, (const uint8_t *)key, (const uint8_t *)seed,
                              partyId, (int)n, (int)m, N, counter));
         queue_counter_release(st, counter);
+    }
+    else if (g == 1 && dpf_quad_lanes(N))
+    {
+        FSS_DISPATCH(es, dpf_eval_quad_kernel<T><<<FSS_SHAPE(4 * N), 0, st>>>(
+                             (T *)sharedOut, (const T *)maskedX, (const uint8_t *)key, (const uint8_t *)seed,
+                             partyId, (int)n, (int)m, N));
     }
     else if (g == 1)
     {

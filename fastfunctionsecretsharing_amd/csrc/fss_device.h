@@ -247,6 +247,89 @@ __device__ __forceinline__ void aes_last(const AesCtx &c, uint32_t s[4], const u
     s[0] = n0, s[1] = n1, s[2] = n2, s[3] = n3;
 }
 
+// ---- column-parallel AES on a quad of lanes ----
+//
+// Lane q (0..3) of a quad holds column q of the state and of the round key.
+// A T-table round needs byte k of column q+k (ShiftRows); those columns come
+// from the quad's other lanes through DPP quad permutations (VALU operand
+// moves, no LDS), so each lane issues 4 table lookups per round instead of
+// 16, and the key schedule's SubWord splits into one S-box lookup per lane.
+
+// DPP quad_perm controls: lane q reads lane sel[q] of its quad
+#define FSS_QP(a, b, c, d) ((a) | ((b) << 2) | ((c) << 4) | ((d) << 6))
+
+template <int CTRL>
+__device__ __forceinline__ uint32_t quad_mov(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, false);
+}
+
+struct QuadAes
+{
+    uint32_t ksAddr; // this lane's key-schedule table offset and byte selector
+    uint32_t ksSel;
+    uint32_t ksMask; // 0xff << 8q
+    uint32_t m1, m2; // all ones when q >= 1, q >= 2 (prefix XOR of the key words)
+
+    __device__ __forceinline__ void init(const AesCtx &c, uint32_t q)
+    {
+        // byte q of SubWord(RotWord(w3)) is S[w3 byte (q+1)&3], read from the
+        // rotated table whose entry holds S in byte q: TE_{(q+3)&3}
+        const uint32_t t = (q + 3u) & 3u;
+        ksAddr           = t == 0 ? c.l[0] : t == 1 ? c.l[1] : t == 2 ? c.l[2] : c.l[3];
+        ksSel            = 0x0C020000u | ((4u + ((q + 1u) & 3u)) << 8);
+        ksMask           = 0xFFu << (8u * q);
+        m1               = q >= 1 ? ~0u : 0u;
+        m2               = q >= 2 ? ~0u : 0u;
+    }
+
+    // this lane's column of the next round key
+    __device__ __forceinline__ uint32_t next_key(const AesCtx &c, uint32_t k, uint32_t rcon) const
+    {
+        const uint32_t w3 = quad_mov<FSS_QP(3, 3, 3, 3)>(k);
+        uint32_t       sb = *(const uint32_t *)(c.base + perm(w3, ksAddr, ksSel)) & ksMask;
+        sb |= quad_mov<FSS_QP(1, 0, 3, 2)>(sb);
+        sb |= quad_mov<FSS_QP(2, 3, 0, 1)>(sb);
+        // k'_q = SubWord(RotWord(k_3)) ^ rcon ^ k_0 ^ ... ^ k_q
+        uint32_t p = k;
+        p ^= quad_mov<FSS_QP(0, 0, 1, 2)>(p) & m1;
+        p ^= quad_mov<FSS_QP(0, 0, 0, 1)>(p) & m2;
+        return xor3(p, sb, rcon);
+    }
+
+    __device__ __forceinline__ static uint32_t round(const AesCtx &c, uint32_t s, uint32_t k)
+    {
+        const uint32_t a1 = quad_mov<FSS_QP(1, 2, 3, 0)>(s);
+        const uint32_t a2 = quad_mov<FSS_QP(2, 3, 0, 1)>(s);
+        const uint32_t a3 = quad_mov<FSS_QP(3, 0, 1, 2)>(s);
+        return xor3(te<0>(c, s, 0), te<1>(c, a1, 1), xor3(te<2>(c, a2, 2), te<3>(c, a3, 3), k));
+    }
+
+    __device__ __forceinline__ static uint32_t last(const AesCtx &c, uint32_t s, uint32_t k)
+    {
+        const uint32_t a1 = quad_mov<FSS_QP(1, 2, 3, 0)>(s);
+        const uint32_t a2 = quad_mov<FSS_QP(2, 3, 0, 1)>(s);
+        const uint32_t a3 = quad_mov<FSS_QP(3, 0, 1, 2)>(s);
+        return aes_last_col(c, s, a1, a2, a3, k);
+    }
+
+    // AES-128 of this lane's plaintext column under the key column `key`
+    __device__ __forceinline__ uint32_t encrypt(const AesCtx &c, uint32_t key, uint32_t pt) const
+    {
+        const uint32_t rcon[10] = {0x01, 0x02, 0x04, 0x08, 0x10, 0x20, 0x40, 0x80, 0x1b, 0x36};
+        uint32_t       k        = key;
+        uint32_t       s        = pt ^ k;
+#pragma unroll
+        for (int r = 0; r < 9; r++)
+        {
+            k = next_key(c, k, rcon[r]);
+            s = round(c, s, k);
+        }
+        k = next_key(c, k, rcon[9]);
+        return last(c, s, k);
+    }
+};
+
 // Encrypt NB blocks (in place) under the 16-byte key `key`, key schedule on the fly.
 template <int NB>
 __device__ __forceinline__ void aes_encrypt(const AesCtx &c, const uint32_t key[4], uint32_t st[NB][4])
