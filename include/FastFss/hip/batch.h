@@ -79,14 +79,15 @@ FAST_FSS_API uint32_t FastFss_hip_batchGetDefaultTile(int op, size_t elementSize
 FAST_FSS_API int FastFss_hip_batchSetBlocksPerCu(int blocks);
 
 /* Groupsize-1 DCF eval kernel variant used by FastFss_cuda_dcfEval: -1 =
- * the per-element-size table, else the default (variant 7 up to 256 elements
- * per CU, variant 2 up to 2^20 elements, variant 6 from there); 0 = per-level
- * key loads; 1..4 = burst loads of CH levels with E elements per lane
- * {(4,1), (8,1), (4,2), (2,2)}, static grid split; 5 = 8-level bursts on the
- * work queue; 6 = variant 5 with each element's value corrections loaded when
- * its walk starts (elements of at most 32 bits; 64-bit elements run variant
- * 5); 7 = two lanes per element, one AES block each per level.  Every variant
- * gives the same shares. */
+ * the per-element-size table, else the default (variant 8 up to 64 elements
+ * per CU, variant 7 up to 256, variant 2 up to 2^20 elements, variant 6 from
+ * there); 0 = per-level key loads; 1..4 = burst loads of CH levels with E
+ * elements per lane {(4,1), (8,1), (4,2), (2,2)}, static grid split; 5 =
+ * 8-level bursts on the work queue; 6 = variant 5 with each element's value
+ * corrections loaded when its walk starts (elements of at most 32 bits; 64-bit
+ * elements run variant 5); 7 = two lanes per element, one AES block each per
+ * level; 8 = eight lanes per element, one column of one AES block each.
+ * Every variant gives the same shares. */
 FAST_FSS_API int FastFss_hip_setEvalVariant(int variant);
 FAST_FSS_API int FastFss_hip_getEvalVariantCount(void);
 FAST_FSS_API int FastFss_hip_setEvalVariantForElementSize(size_t elementSize, int variant);
