@@ -120,6 +120,58 @@ __global__ __launch_bounds__(kBlock) void mic_eval_small_kernel(T *out, const T 
     mic_eval_body<T>(c, out, x, key, z, seed, party, left, right, I, blocksPerElem, n, outBytes, N, counter);
 }
 
+// mic_eval_range with each DCF walk on eight lanes (dcf_eval_g1_oct), for
+// calls small enough that eight lanes per (element, block of intervals) still
+// leave at most two waves per SIMD.  Lane 4 of each group of eight writes the
+// outputs.  Same results as mic_eval_kernel.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void mic_eval_oct_kernel(T *out, const T *x, const uint8_t *key, const T *z,
+                                                              const uint8_t *seed, int party, const T *left,
+                                                              const T *right, size_t I, size_t blocksPerElem, int n,
+                                                              int outBytes, size_t N)
+{
+    FSS_KERNEL_PROLOGUE_STAGED
+    const uint32_t q = threadIdx.x & 3u, h = (threadIdx.x >> 2) & 1u;
+    const bool     writer = (threadIdx.x & 7u) == 4u;
+    QuadAes        qa;
+    qa.init(c, q);
+    const T     *tails = (const T *)(key + N * 16 * (size_t)n);
+    const size_t span  = (I + blocksPerElem - 1) / blocksPerElem;
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; (g >> 3) < N * blocksPerElem;
+         g += (size_t)gridDim.x * blockDim.x)
+    {
+        const size_t w = g >> 3, e = w / blocksPerElem, first = (w % blocksPerElem) * span;
+        if (first >= I) continue;
+        const size_t   cnt = (first + span < I) ? span : (I - first);
+        const uint32_t s   = *(const uint32_t *)(seed + 16 * e + 4 * q);
+        const uint8_t *sc  = key + e * 16 * (size_t)n;
+        const T       *tl  = tails + e * (size_t)(n + 3);
+        const T        xe  = mod_bits<T>(x[e], n);
+        T              sp = 0, sq = 0;
+        for (size_t i = first; i < first + cnt; i++)
+        {
+            const T qP  = mod_bits<T>((T)(right[i] + 1), n);
+            const T xP  = mod_bits<T>((T)(xe - 1 - left[i]), n);
+            const T xQp = mod_bits<T>((T)(xe - 1 - (T)(right[i] + 1)), n);
+            const T p   = mod_bits<T>(left[i], n);
+            if (i > first && left[i] == mod_bits<T>((T)(right[i - 1] + 1), n))
+                sp = sq;
+            else
+                sp = dcf_eval_g1_oct<T>(c, qa, sc, tl, xP, s, party, n, outBytes, h, q);
+            sq  = dcf_eval_g1_oct<T>(c, qa, sc, tl, xQp, s, party, n, outBytes, h, q);
+            T o = (T)(sq - sp + z[e * I + i]);
+            if (party == 1) o = (T)(o + (T)((int)(xe > p) - (int)(xe > qP)));
+            if (writer) out[e * I + i] = o;
+        }
+    }
+}
+
+// eight lanes per work item while they leave at most two waves per SIMD
+static bool mic_oct_lanes(size_t items)
+{
+    return 8 * items <= (size_t)device_cu_count() * 512;
+}
+
 static size_t mic_blocks_per_elem(size_t N, size_t I)
 {
     // split each element's intervals until the launch has a lane per CU slot
@@ -135,7 +187,17 @@ int launch_mic_eval_rt(size_t es, int grid, hipStream_t st, void *out, const voi
                        size_t N, uint32_t *counter, int block)
 {
     const size_t bpe = mic_blocks_per_elem(N, I);
-    if (block < kBlock)
+    // (block < kBlock: the per-call shape of a small call; the serving
+    // runtime's one-workgroup-per-1024-elements shape keeps its launch)
+    if (!counter && block < kBlock && mic_oct_lanes(N * bpe))
+    {
+        // its own launch shape: eight lanes per work item
+        const LaunchShape os = shape_for(8 * N * bpe);
+        FSS_DISPATCH(es, mic_eval_oct_kernel<T><<<os.grid, os.block, 0, st>>>(
+                             (T *)out, (const T *)x, (const uint8_t *)key, (const T *)z, (const uint8_t *)seed, party,
+                             (const T *)left, (const T *)right, I, bpe, n, (m + 7) / 8, N));
+    }
+    else if (block < kBlock)
     {
         FSS_DISPATCH(es, mic_eval_small_kernel<T><<<grid, block, 0, st>>>(
                              (T *)out, (const T *)x, (const uint8_t *)key, (const T *)z, (const uint8_t *)seed, party,
