@@ -381,12 +381,14 @@ int launch_dcf_eval_g1(int v, int grid, int block, hipStream_t st, T *out, const
 }
 
 // Serving launches choose their own grid shape, so they keep the static split
-// (variants 5 to 8 fall back to variant 2 there).
+// (variants 5 to 8 fall back to variant 2 there), except that the per-call
+// shape of a small call (block < kBlock) takes the lane-pair and eight-lane
+// variants with their own shapes, as the per-call entry point does.
 int launch_dcf_eval_g1_rt(size_t es, int grid, hipStream_t st, void *out, const void *x, const void *key,
                           const void *seed, int party, int n, int m, size_t N, int block)
 {
     int v = dcf_eval_variant(es, (size_t)n, N);
-    if (v >= 5) v = 2;
+    if (v == 5 || v == 6 || ((v == 7 || v == 8) && block >= kBlock)) v = 2;
     int rc = FAST_FSS_SUCCESS;
     FSS_DISPATCH(es, rc = launch_dcf_eval_g1<T>(v, grid, block, st, (T *)out, (const T *)x, (const uint8_t *)key,
                                                 (const uint8_t *)seed, party, n, (m + 7) / 8, N));

@@ -639,9 +639,24 @@ inline void full_domain_split(int lv, size_t N, int &L, int &D)
     D = lv - L;
 }
 
+// Four lanes per element while they leave at most two waves per SIMD.
+static bool dpf_quad_lanes(size_t N)
+{
+    return 4 * N <= (size_t)device_cu_count() * 512;
+}
+
 int launch_dpf_eval_g1_rt(size_t es, int grid, hipStream_t st, void *out, const void *x, const void *key,
                           const void *seed, int party, int n, int m, size_t N, int block)
 {
+    // the per-call shape of a small call (block < kBlock) takes the per-call
+    // entry point's four-lane kernel; one workgroup per 1024 elements keeps
+    // the given launch
+    if (block < kBlock && dpf_quad_lanes(N))
+    {
+        FSS_DISPATCH(es, dpf_eval_quad_kernel<T><<<FSS_SHAPE(4 * N), 0, st>>>(
+                             (T *)out, (const T *)x, (const uint8_t *)key, (const uint8_t *)seed, party, n, m, N));
+        return launch_status();
+    }
     FSS_DISPATCH(es, dpf_eval_burst_kernel<T, 8><<<grid, block, 0, st>>>((T *)out, (const T *)x,
                                                                            (const uint8_t *)key,
                                                                            (const uint8_t *)seed, party, n, m, N));
@@ -702,12 +717,6 @@ static int dpf_eval_head(int party, size_t n, size_t m, size_t g, size_t es)
     if (g == 0) return FAST_FSS_INVALID_GROUP_SIZE_ERROR;
     if (party != 0 && party != 1) return FAST_FSS_INVALID_PARTY_ID_ERROR;
     return FAST_FSS_SUCCESS;
-}
-
-// Four lanes per element while they leave at most two waves per SIMD.
-static bool dpf_quad_lanes(size_t N)
-{
-    return 4 * N <= (size_t)device_cu_count() * 512;
 }
 
 extern "C" FAST_FSS_API int FastFss_cuda_dpfEval(void *sharedOut, size_t sharedOutDataSize, const void *maskedX,
