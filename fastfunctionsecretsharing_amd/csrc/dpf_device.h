@@ -295,4 +295,50 @@ __device__ __forceinline__ T grotto_eval_one(const AesCtx &c, const uint8_t *sCW
     return (T)(parity ^ (uint32_t)party);
 }
 
+// grotto_eval_one (mode 0 lt, 1 le) and grotto_eq_eval_one (mode 2) on a quad
+// of lanes per element, one column of the seed block per lane (QuadAes), for
+// calls too small to fill the GPU.  Lane 0 of the quad returns the share;
+// every lane of the quad takes the same branches (one element).
+template <typename T>
+__device__ __forceinline__ T grotto_eval_quad(const AesCtx &c, const QuadAes &qa, const uint8_t *sCW, uint64_t lastCW,
+                                              T tl, T tr, T x, uint32_t s, int party, int n, int mode, uint32_t q)
+{
+    x = mod_bits<T>(x, n);
+    if (q == 0) s &= ~1u;
+    uint32_t   t = (uint32_t)party, parity = 0;
+    const bool equalBound = mode == 1;
+    const T    path       = mode == 2 ? (T)(x >> 6) : x; // eq walks the leaf index, lt/le x's top bits
+    const int  top        = mode == 2 ? n - 6 : n;
+    for (int i = 0; i < n - 6; i++)
+    {
+        // the remaining low n-i bits all zero: x is on the left spine below
+        if (equalBound && mod_bits<T>(x, n - i) == 0) return (T)(parity ^ (uint32_t)party);
+        const uint32_t bit  = (uint32_t)(path >> (top - 1 - i)) & 1u;
+        const uint32_t prev = t;
+        const uint32_t cw   = *(const uint32_t *)(sCW + 16 * i + 4 * q);
+        // plaintext columns (2 bit, 0, 2 bit + 1, 0)
+        const uint32_t pt  = (q & 1u) ? 0u : 2u * bit + (q >> 1);
+        const uint32_t blk = qa.encrypt(c, s, pt);
+        const uint32_t c0  = quad_mov<FSS_QP(0, 0, 0, 0)>(blk);
+        const uint32_t mk  = 0u - t;
+        uint32_t       tn  = c0 & 1u;
+        s                  = ((q == 0) ? (blk & ~1u) : blk) ^ (cw & mk);
+        tn ^= (uint32_t)(((bit ? tr : tl) >> i) & 1) & t;
+        t = tn;
+        parity ^= bit & (prev ^ t);
+    }
+    // the leaf word: seed columns 2 and 3
+    const uint32_t s2 = quad_mov<FSS_QP(2, 2, 2, 2)>(s), s3 = quad_mov<FSS_QP(3, 3, 3, 3)>(s);
+    const uint64_t u  = ((uint64_t)s2 | ((uint64_t)s3 << 32)) ^ (t ? lastCW : 0ull);
+    const uint32_t p  = (uint32_t)((uint64_t)x & 63);
+    if (mode == 2) return (T)((u >> p) & 1ull);
+    if (equalBound)
+    {
+        if (p != 0) parity ^= parity64(u & (~0ull >> (64 - p)));
+    }
+    else
+        parity ^= parity64(u & (~0ull >> (63 - p)));
+    return (T)(parity ^ (uint32_t)party);
+}
+
 } // namespace fss

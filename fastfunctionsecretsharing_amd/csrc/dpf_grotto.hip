@@ -191,6 +191,29 @@ __global__ __launch_bounds__(kBlock) void grotto_eval_kernel(T *out, const T *x,
     }
 }
 
+// Four lanes per element (grotto_eval_quad); the grid-stride step is a
+// multiple of 4, so a quad always holds one element.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void grotto_eval_quad_kernel(T *out, const T *x, const uint8_t *key,
+                                                                  const uint8_t *seed, int party, int n, int mode,
+                                                                  size_t N)
+{
+    FSS_KERNEL_PROLOGUE_STAGED
+    const uint32_t q = threadIdx.x & 3u;
+    QuadAes        qa;
+    qa.init(c, q);
+    const GrottoKeyPtrs k   = grotto_key(key, n, N);
+    const T            *tlr = (const T *)k.tlr;
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; (g >> 2) < N; g += (size_t)gridDim.x * blockDim.x)
+    {
+        const size_t   e  = g >> 2;
+        const uint32_t s  = *(const uint32_t *)(seed + 16 * e + 4 * q);
+        const uint8_t *sc = k.sCW + e * 16 * (size_t)(n - 6);
+        const T v = grotto_eval_quad<T>(c, qa, sc, k.lastCW[e], tlr[2 * e], tlr[2 * e + 1], x[e], s, party, n, mode, q);
+        if (q == 0) out[e] = v;
+    }
+}
+
 // Interval shares of one element (src/impl/grotto.h:393-447); writes `I`
 // boolean shares and, for the LUT form, also accumulates E and T.
 template <typename T>
@@ -897,8 +920,18 @@ static int grotto_eval_launch(void *out, size_t outSize, const void *x, size_t x
         return wide_grotto_eval(st, 0, mode, out, nullptr, nullptr, x, key, seed, party, (int)n, es, N, nullptr,
                                 nullptr, 0, nullptr, 0);
     }
-    FSS_DISPATCH(es, grotto_eval_kernel<T><<<FSS_SHAPE(N), 0, st>>>(
-                         (T *)out, (const T *)x, (const uint8_t *)key, (const uint8_t *)seed, party, (int)n, mode, N));
+    if (dpf_quad_lanes(N))
+    {
+        FSS_DISPATCH(es, grotto_eval_quad_kernel<T><<<FSS_SHAPE(4 * N), 0, st>>>(
+                             (T *)out, (const T *)x, (const uint8_t *)key, (const uint8_t *)seed, party, (int)n, mode,
+                             N));
+    }
+    else
+    {
+        FSS_DISPATCH(es, grotto_eval_kernel<T><<<FSS_SHAPE(N), 0, st>>>(
+                             (T *)out, (const T *)x, (const uint8_t *)key, (const uint8_t *)seed, party, (int)n, mode,
+                             N));
+    }
     return launch_status();
 }
 
