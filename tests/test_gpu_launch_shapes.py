@@ -137,6 +137,35 @@ def test_spline_split_shapes(dev, N):
     assert np.array_equal(rec, spline_closed_form(masked, alpha, left, right, coeff, d, n, m))
 
 
+@pytest.mark.parametrize("n,m,es,I,d,N", [(24, 32, 4, 16, 2, 17), (24, 32, 4, 16, 2, 1000), (24, 32, 4, 16, 2, 4096),
+                                          (32, 32, 4, 4, 2, 8192), (24, 32, 4, 16, 2, 8192), (40, 32, 8, 5, 3, 1000),
+                                          (20, 30, 4, 6, 1, 300)])
+def test_spline_small_call_lanes(dev, n, m, es, I, d, N):
+    """Small spline calls with 4-byte slots walk each DCF point on eight lanes (column-parallel AES)
+    with 8-, 4- and 2-way interval splits at these sizes (16 intervals at 8192 elements would be 8
+    per item, so that call takes the split kernel); degree 2 makes linked walks that touch three
+    conversion windows, 64-bit elements carry 4-byte outputs, and interval 3 starts after a gap.
+    Keys of a sample equal the oracle's, both parties' shares of the sample equal the oracle's, and
+    every element reconstructs to its closed form."""
+    rng = np.random.default_rng(N + 7 * n + d)
+    left, right, coeff, alpha, masked, seeds, kd, es_, bs = _spline_setup(rng, N, n, m, es, I, d)
+    g = I * (d + 1)
+    idx = np.unique(np.concatenate([rng.choice(N, min(N, 200), replace=False), np.arange(I + 1), [N - 1]]))
+    ko = spline_keygen(OR, alpha[idx], seeds[0].reshape(N, 16)[idx].ravel(), seeds[1].reshape(N, 16)[idx].ravel(),
+                       coeff, d, left, right, n, m, es)[0]
+    assert np.array_equal(sub_key_rows(dev.get(kd), idx, 16 * n, (n * g + 2 + g) * es, N), ko)
+    ys = []
+    for party in (0, 1):
+        y = dev.get(spline_eval(dev, masked, kd, es_[party], bs[party], seeds[party], party, left, right, d, n, m, es))
+        yo = spline_eval(OR, masked[idx], ko, es_[party].reshape(N, g)[idx].ravel(),
+                         bs[party].reshape(N, g)[idx].ravel(), seeds[party].reshape(N, 16)[idx].ravel(), party, left,
+                         right, d, n, m, es)
+        assert np.array_equal(y[idx], yo), party
+        ys.append(y)
+    rec = (ys[0].astype(np.uint64) + ys[1]) & np.uint64(mask(m))
+    assert np.array_equal(rec, spline_closed_form(masked, alpha, left, right, coeff, d, n, m))
+
+
 @pytest.mark.parametrize("n,m,es", [(24, 32, 4), (12, 10, 2)])
 @pytest.mark.parametrize("N", [1000, 300000])
 def test_spline_reconstruction_in_ne_out(dev, n, m, es, N):
