@@ -285,9 +285,13 @@ __global__ __launch_bounds__(kBlock) void grotto_lut_kernel(T *outE, T *outT, T 
 
 // ------------------------------------------------------ full-domain Grotto ----
 //
-// Lane (e, r) owns the subtree r (0 <= r < 2^L) of key e: it walks the L top
-// levels along r (one key schedule + one block per level, as the per-point
-// eval does), then expands the remaining D = n-6-L levels depth-first, each
+// Lane (e, r) owns the subtree r (0 <= r < 2^L) of key e.  The workgroup
+// first computes the top nodes its lanes share, breadth first through LDS:
+// depth d (1 <= d < L) has one node per 2^(L-d) lanes, each made by one thread
+// from its parent (one key schedule + one block, as the per-point eval's
+// step), so a node is computed once rather than by each of its lanes.  Each
+// lane then takes the step to its own depth-L node, and expands the remaining
+// D = n-6-L levels depth-first, each
 // inner node giving both children from one key schedule and two blocks.  The
 // pending right children sit in a register stack; every lane of a wave visits
 // its leaves in the same order, so the stack depth is wave-uniform and the
@@ -361,28 +365,81 @@ __global__ __launch_bounds__(kBlock) void grotto_full_kernel(void *out, const ui
     const size_t        lanes = N << L;
     const int           lv    = n - 6; // tree levels above the leaf words
     const uint32_t      lane  = threadIdx.x & 63u;
-    // whole waves iterate together (the element layout's stores are shared
-    // across the wave); a lane past the end redoes its wave's first subtree
-    // and writes nothing of its own
-    for (size_t wb = (size_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); wb < lanes;
-         wb += (size_t)gridDim.x * blockDim.x)
+    // nodes of two consecutive top depths (at most blockDim/2 + 1 a depth):
+    // seed words and t | parity << 1
+    __shared__ uint32_t topS[2][4][kBlock / 2 + 1];
+    __shared__ uint32_t topTP[2][kBlock / 2 + 1];
+    // the workgroup iterates together (its lanes share top nodes); whole waves
+    // work together (the element layout's stores are shared across the
+    // wave); a lane past the end redoes its wave's first subtree and writes
+    // nothing of its own
+    for (size_t base = (size_t)blockIdx.x * blockDim.x; base < lanes; base += (size_t)gridDim.x * blockDim.x)
     {
-        const bool     valid = wb + lane < lanes;
-        const size_t   w     = valid ? wb + lane : wb;
+        const size_t end = base + blockDim.x < lanes ? base + blockDim.x : lanes;
+        for (int d = 1; d < L; d++)
+        {
+            // node u of depth d is the ancestor of lanes u << (L-d) ..
+            const size_t u0 = base >> (L - d), cnt = ((end - 1) >> (L - d)) - u0 + 1;
+            if (threadIdx.x < cnt)
+            {
+                const size_t u = u0 + threadIdx.x, e = u >> d;
+                uint32_t     s[4], t, P;
+                if (d == 1)
+                {
+                    load16(s, seed + 16 * e);
+                    s[0] &= ~1u;
+                    t = (uint32_t)party;
+                    P = 0;
+                }
+                else
+                {
+                    const uint32_t j = (uint32_t)((u >> 1) - (base >> (L - d + 1)));
+#pragma unroll
+                    for (int q = 0; q < 4; q++) s[q] = topS[(d - 1) & 1][q][j];
+                    t = topTP[(d - 1) & 1][j] & 1u;
+                    P = topTP[(d - 1) & 1][j] >> 1;
+                }
+                const uint32_t bit = (uint32_t)u & 1u, prev = t;
+                tree_eval_step<T>(c, s, t, bit, k.sCW + e * 16 * (size_t)lv + 16 * (d - 1), tlr[2 * e],
+                                  tlr[2 * e + 1], d - 1);
+                P ^= bit & (prev ^ t);
+#pragma unroll
+                for (int q = 0; q < 4; q++) topS[d & 1][q][threadIdx.x] = s[q];
+                topTP[d & 1][threadIdx.x] = t | (P << 1);
+            }
+            __syncthreads();
+        }
+        const size_t   wb    = base + (threadIdx.x & ~63u);
+        const bool     valid = base + threadIdx.x < lanes;
+        const size_t   w     = valid ? base + threadIdx.x : (wb < lanes ? wb : base);
         const size_t   e     = w >> L;
         const uint32_t r     = (uint32_t)(w & ((1u << L) - 1u));
-        const uint8_t *sc = k.sCW + e * 16 * (size_t)lv;
+        const uint8_t *sc    = k.sCW + e * 16 * (size_t)lv;
+        uint32_t       s[4], t, P;
+        if (L < 2)
+        {
+            load16(s, seed + 16 * e);
+            s[0] &= ~1u;
+            t = (uint32_t)party;
+            P = 0;
+        }
+        else
+        {
+            const uint32_t j = (uint32_t)((w >> 1) - (base >> 1));
+#pragma unroll
+            for (int q = 0; q < 4; q++) s[q] = topS[(L - 1) & 1][q][j];
+            t = topTP[(L - 1) & 1][j] & 1u;
+            P = topTP[(L - 1) & 1][j] >> 1;
+        }
+        // the next iteration's top nodes overwrite these buffers
+        if (L >= 2) __syncthreads();
+        if (wb >= lanes) continue;
         const T        tl = tlr[2 * e], tr = tlr[2 * e + 1];
         const uint64_t lastCW = k.lastCW[e];
-        uint32_t       s[4];
-        load16(s, seed + 16 * e);
-        s[0] &= ~1u;
-        uint32_t t = (uint32_t)party, P = 0;
-        for (int i = 0; i < L; i++)
+        if (L >= 1)
         {
-            const uint32_t bit  = (r >> (L - 1 - i)) & 1u;
-            const uint32_t prev = t;
-            tree_eval_step<T>(c, s, t, bit, sc + 16 * i, tl, tr, i);
+            const uint32_t bit = r & 1u, prev = t;
+            tree_eval_step<T>(c, s, t, bit, sc + 16 * (L - 1), tl, tr, L - 1);
             P ^= bit & (prev ^ t);
         }
 

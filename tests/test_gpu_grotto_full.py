@@ -110,11 +110,14 @@ def test_full_domain_errors(dev):
     assert call(None, 0, None, 0, None, 0, 0, LT, 0, n, es, 0, sp) == 0
 
 
-def test_full_domain_4096_keys_n16(dev):
-    """Grotto at bitWidthIn = 16 over 4096 keys, full domain 2^16 points."""
+@pytest.mark.parametrize("N", [4096, 4095])
+def test_full_domain_4096_keys_n16(dev, N):
+    """Grotto at bitWidthIn = 16 over 4096 keys, full domain 2^16 points.  At 4095 keys each lane
+    owns a subtree of depth 3 below 7 shared top levels, and a workgroup makes two passes whose
+    key groups straddle workgroup boundaries."""
     torch = dev.torch
-    n, es, N = 16, 2, 4096
-    rng = np.random.default_rng(4096)
+    n, es = 16, 2
+    rng = np.random.default_rng(N)
     alpha = rand_elems(rng, N, n, es)
     alpha[:3] = [0, 1, 0xFFFF]
     s0, s1 = rand_seeds(rng, N), rand_seeds(rng, N)
@@ -137,7 +140,7 @@ def test_full_domain_4096_keys_n16(dev):
     # element layout on a sample of keys equals the per-point device evaluation
     ge = _full(dev, kd, s0, 0, n, es, N, LT, 0).reshape(N, 1 << n)
     key_h = dev.get(kd)
-    for e in (0, 1, 2, 1234, 4095):
+    for e in (0, 1, 2, 1234, N - 1):
         idx = np.full(1 << n, e)
         k1 = dev.put(_replicate(key_h, n, es, N, idx))
         s = s0.reshape(N, 16)[idx].reshape(-1)
