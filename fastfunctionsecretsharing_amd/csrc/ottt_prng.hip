@@ -118,7 +118,8 @@ __device__ inline void ottt_stage(T *tab, const T *lut, int t0, int tb, uint32_t
         tab[i] = lut[(size_t)(t0 + (int)(i >> logC)) * total + c0 + (i & (C - 1))];
 }
 
-// v : 0 per lane by the lane's bit of a wave-uniform 64-bit mask (one VALU op)
+// v : 0 per lane by the lane's bit of a wave-uniform 64-bit mask in SGPRs
+// (one VALU op)
 __device__ inline uint32_t ottt_select(uint64_t m, uint32_t v)
 {
     uint32_t r;
@@ -136,16 +137,16 @@ __device__ inline A ottt_select_acc(uint64_t m, A v)
 }
 
 // Steps over nwords (a multiple of U) consecutive key words of one element,
-// lane (k0 + s) & kmask of kv holding word s (broadcast with readlane).  Lane
-// L's table entry for bit L of word s sits at j0 - 64 s of the staged table;
-// the U reads of a group are issued, at immediate offsets from one address,
-// before any of their adds.  FULL: every set bit counts, and the word itself
-// is the select's lane mask; otherwise the first and last words are masked
-// to the staged run (firstMaskShift, lastMaskBits).
+// lane (k0 + s) & kmask of kv holding word s (broadcast with readlane, so each
+// word is wave-uniform and serves as its select's lane mask).  Lane L's table
+// entry for bit L of word s sits at j0 - 64 s of the staged table; the U reads
+// of a group are issued, at immediate offsets from one address, before any of
+// their adds.  FULL: every set bit counts; otherwise the first and last words
+// are masked to the staged run (firstMaskShift, lastMaskBits).
 template <int U, int TB, bool FULL, typename T, typename A>
 __device__ inline void ottt_words(A (&acc)[kOtttTables], uint32_t &E, const T *tab, uint32_t stride, uint64_t kv,
                                   uint32_t k0, uint32_t kmask, uint32_t nwords, uint32_t j0, uint32_t firstMaskShift,
-                                  uint32_t lastMaskBits, uint32_t lane)
+                                  uint32_t lastMaskBits)
 {
     for (uint32_t s0 = 0; s0 < nwords; s0 += U)
     {
@@ -169,9 +170,16 @@ __device__ inline void ottt_words(A (&acc)[kOtttTables], uint32_t &E, const T *t
         for (int u = 0; u < U; u++)
         {
             E += (uint32_t)__popcll(word[u]);
-            const bool take = (word[u] >> lane) & 1; // used where the masked word may sit in vector registers
+            // Either way the word is the select's lane mask.  FULL: the word
+            // straight from readlane, selected by the asm form (at n = 12 the
+            // builtin below ran 6% slower, at n = 8 the same); masked words:
+            // the builtin, which leaves the compiler free to place the masked
+            // word (the per-lane bit extraction used before cost 27% at
+            // n = 16, same-process A/B with equal outputs)
+            const bool take = __builtin_amdgcn_inverse_ballot_w64(word[u]);
 #pragma unroll
-            for (int t = 0; t < TB; t++) acc[t] += FULL ? ottt_select_acc<A>(word[u], v[u][t]) : (take ? v[u][t] : (A)0);
+            for (int t = 0; t < TB; t++)
+                acc[t] += FULL ? ottt_select_acc<A>(word[u], v[u][t]) : (take ? v[u][t] : (A)0);
         }
     }
 }
@@ -243,7 +251,7 @@ __global__ __launch_bounds__(256) void ottt_eval_tile_kernel(T *__restrict__ out
         {
             const uint32_t j0 = ((ottt_readlane(xv, 0) - lane) & mask) + total;
             ottt_words<16, TB, true, T, A>(acc, E, tab, 2 * total, kv, 0, 63, 64,
-                                                j0 - 64u * 64u * blk, 0, 64, lane);
+                                                j0 - 64u * 64u * blk, 0, 64);
             if (blk == B - 1)
             {
 #pragma unroll
@@ -263,17 +271,13 @@ __global__ __launch_bounds__(256) void ottt_eval_tile_kernel(T *__restrict__ out
                 if (e >= N) break;
                 const uint32_t j0 = ((ottt_readlane(xv, le) - lane) & mask) + total;
                 if (W >= 8)
-                    ottt_words<8, TB, true, T, A>(acc, E, tab, 2 * total, kv, le * W, 63, W, j0, 0, 64,
-                                                        lane);
+                    ottt_words<8, TB, true, T, A>(acc, E, tab, 2 * total, kv, le * W, 63, W, j0, 0, 64);
                 else if (W == 4)
-                    ottt_words<4, TB, true, T, A>(acc, E, tab, 2 * total, kv, le * W, 63, W, j0, 0, 64,
-                                                        lane);
+                    ottt_words<4, TB, true, T, A>(acc, E, tab, 2 * total, kv, le * W, 63, W, j0, 0, 64);
                 else if (W == 2)
-                    ottt_words<2, TB, true, T, A>(acc, E, tab, 2 * total, kv, le * W, 63, W, j0, 0, 64,
-                                                        lane);
+                    ottt_words<2, TB, true, T, A>(acc, E, tab, 2 * total, kv, le * W, 63, W, j0, 0, 64);
                 else
-                    ottt_words<1, TB, true, T, A>(acc, E, tab, 2 * total, kv, le * W, 63, W, j0, 0, 64,
-                                                        lane);
+                    ottt_words<1, TB, true, T, A>(acc, E, tab, 2 * total, kv, le * W, 63, W, j0, 0, 64);
 #pragma unroll
                 for (int t = 0; t < kOtttTables; t++)
                     if (t < TB) acc[t] = ottt_wave_sum(acc[t]);
@@ -351,11 +355,11 @@ __global__ __launch_bounds__(1024) void ottt_eval_chunked_kernel(T *__restrict__
                     const uint32_t cnt = min(64u, nw - 1 - blk * 64);
                     const uint64_t kv  = lane < cnt ? ottt_load_word(kb, (k0 + blk * 64 + lane) & (W - 1)) : 0;
                     ottt_words<8, TB, false, T, A>(acc[w], E[w], tab, stride, kv, 0, 63, cnt,
-                                                         j0 - 64u * 64u * blk, blk == 0 ? off : 0, 64, lane);
+                                                         j0 - 64u * 64u * blk, blk == 0 ? off : 0, 64);
                 }
                 const uint64_t kl = lane == 0 ? ottt_load_word(kb, (k0 + nw - 1) & (W - 1)) : 0;
                 ottt_words<1, TB, false, T, A>(acc[w], E[w], tab, stride, kl, 0, 63, 1,
-                                                     j0 - 64u * (nw - 1), 0, off, lane);
+                                                     j0 - 64u * (nw - 1), 0, off);
             }
         }
 #pragma unroll
