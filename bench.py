@@ -653,14 +653,22 @@ def serving_capacity(templates, picks, N=1 << 20, reps=3):
         t = W.make_template(op, n, N, rng, gen, dev, sp)
         out = torch.empty(t.out_elems(), dtype=W.torch_dtype(t.es), device=dev)
         cache = W.template_spline_cache(t) if t.degree else None
-        W.eval_template_alone(t, out, sp, cache)
+        # two warm-up calls (the first class timed in a process read up to 15% low after one),
+        # then one timed call sizes the run: at least `reps` calls and about 20 ms of work
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        W.eval_template_alone(t, out, sp, cache)
+        W.eval_template_alone(t, out, sp, cache)
         e0.record(stream)
-        for _ in range(reps):
+        W.eval_template_alone(t, out, sp, cache)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        calls = max(reps, min(40, int(20.0 / max(e0.elapsed_time(e1), 1e-3))))
+        e0.record(stream)
+        for _ in range(calls):
             W.eval_template_alone(t, out, sp, cache)
         e1.record(stream)
         torch.cuda.synchronize()
-        rates[(op, n)] = N * reps / (e0.elapsed_time(e1) / 1e3)
+        rates[(op, n)] = N * calls / (e0.elapsed_time(e1) / 1e3)
         del t, out, cache
         torch.cuda.empty_cache()
     device_s = sum(templates[p].N / rates[(templates[p].op, templates[p].n)] for p in picks)
