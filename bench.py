@@ -41,6 +41,8 @@ TRACE_PROFILE = REPO / "profiles" / "dcf_eval_queue_pre_n32_2p24_kernel_trace.md
 # (probe/lds_probe.hip), recorded here; bench.py also measures it live on the
 # box it runs on and takes the live figure as the roofline peak
 LDS_PEAK_PROFILE = REPO / "profiles" / "lds_peak.md"
+# recorded SQ instruction counts of every bench leg's kernels
+ALL_KERNELS_PROFILE = REPO / "profiles" / "all_kernels_bench_legs.md"
 # probe layouts whose reads are 4-byte lookups (one or two per instruction)
 LDS_LOOKUP_LAYOUTS = (0, 1, 5, 7, 8, 9)
 # Algorithmic AES table lookups.  A key schedule is 10 rounds x 4 S-box
@@ -544,6 +546,12 @@ def ottt_leg(reps: int = 10):
             "lds_lookups_per_element": total * lutNum,
             "lds_frac": lds_frac(lookups / (dev_ms * 1e-3)),
         }
+        # the bound: VALU issue (a select and an add per key word and table, the readlanes that
+        # broadcast the words, the wave sums); counts recorded with rocprofv3 SQ_INSTS_VALU
+        valu = profiled_value(ALL_KERNELS_PROFILE, "ottt_n%d_valu_instructions_per_element" % n)
+        if valu:
+            res["n%d" % n].update({"bound": "valu issue", "valu_issued_per_element": valu,
+                                   "valu_issue_frac": valu * N / (dev_ms * 1e-3) / 1e12 / VALU_PEAK_TOPS})
         del key0, key1, E, T, lut, lut32
         torch.cuda.empty_cache()
     del holder
