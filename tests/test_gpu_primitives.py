@@ -320,7 +320,8 @@ def test_spline(dev, n, m, es, I, d):
 @pytest.mark.parametrize("n,es,lutNum", [(3, 1, 1), (4, 1, 2), (5, 1, 1), (8, 2, 3), (12, 4, 2), (9, 8, 1),
                                          # n = 6: one 64-bit word; tables beyond four: several passes
                                          (6, 1, 2), (10, 4, 5), (7, 8, 9),
-                                         # tables staged in LDS chunks (64 KB each): 4 and 4 chunks
+                                         # tables staged in LDS chunks (80 KB budget): 4 chunks each;
+                                         # n = 14 in 2-byte elements fits whole (64 KB)
                                          (13, 8, 4), (16, 4, 1), (14, 2, 0)])
 def test_ottt(dev, n, es, lutNum):
     rng = np.random.default_rng(n * 5 + lutNum)
