@@ -147,8 +147,10 @@ __device__ inline A ottt_select_acc(uint64_t m, A v)
 // entry for bit L of word s sits at j0 - 64 s of the staged table; the U reads
 // of a group are issued, at immediate offsets from one address, before any of
 // their adds.  FULL: every set bit counts; otherwise the first and last words
-// are masked to the staged run (firstMaskShift, lastMaskBits).
-template <int U, int TB, bool FULL, typename T, typename A>
+// are masked to the staged run (firstMaskShift, lastMaskBits).  COUNT_E: add
+// the words' set bits to E (two scalar instructions a word); callers holding
+// the words in vector registers count them there instead.
+template <int U, int TB, bool FULL, bool COUNT_E, typename T, typename A>
 __device__ inline void ottt_words(A (&acc)[kOtttTables], uint32_t &E, const T *tab, uint32_t stride, uint64_t kv,
                                   uint32_t k0, uint32_t kmask, uint32_t nwords, uint32_t j0, uint32_t firstMaskShift,
                                   uint32_t lastMaskBits)
@@ -174,7 +176,7 @@ __device__ inline void ottt_words(A (&acc)[kOtttTables], uint32_t &E, const T *t
 #pragma unroll
         for (int u = 0; u < U; u++)
         {
-            E += (uint32_t)__popcll(word[u]);
+            if (COUNT_E) E += (uint32_t)__popcll(word[u]);
             // Either way the word is the select's lane mask.  FULL: the word
             // straight from readlane, selected by the asm form (at n = 12 the
             // builtin below ran 6% slower, at n = 8 the same); masked words:
@@ -235,7 +237,8 @@ __global__ __launch_bounds__(256) void ottt_eval_tile_kernel(T *__restrict__ out
     uint64_t kv  = load_block(u, 0);
     uint32_t xv  = load_x(u);
     A        acc[kOtttTables];
-    uint32_t E = 0;
+    uint32_t E  = 0;
+    uint32_t Ev = 0; // W >= 64: set bits of this lane's words of the element
 #pragma unroll
     for (int t = 0; t < kOtttTables; t++) acc[t] = 0;
     while (true)
@@ -255,15 +258,18 @@ __global__ __launch_bounds__(256) void ottt_eval_tile_kernel(T *__restrict__ out
         if (W >= 64)
         {
             const uint32_t j0 = ((ottt_readlane(xv, 0) - lane) & mask) + total;
-            ottt_words<16, TB, true, T, A>(acc, E, tab, 2 * total, kv, 0, 63, 64,
-                                                j0 - 64u * 64u * blk, 0, 64);
+            ottt_words<16, TB, true, false, T, A>(acc, E, tab, 2 * total, kv, 0, 63, 64,
+                                                       j0 - 64u * 64u * blk, 0, 64);
+            Ev += (uint32_t)__popcll(kv);
             if (blk == B - 1)
             {
 #pragma unroll
                 for (int t = 0; t < kOtttTables; t++)
                     if (t < TB) acc[t] = ottt_wave_sum(acc[t]);
+                E = ottt_wave_sum(Ev);
                 if (lane == 0) ottt_write<T, A>(outE, outT, u, lutNum, t0, TB, party, writeE, acc, E);
-                E = 0;
+                E  = 0;
+                Ev = 0;
 #pragma unroll
                 for (int t = 0; t < kOtttTables; t++) acc[t] = 0;
             }
@@ -276,13 +282,13 @@ __global__ __launch_bounds__(256) void ottt_eval_tile_kernel(T *__restrict__ out
                 if (e >= N) break;
                 const uint32_t j0 = ((ottt_readlane(xv, le) - lane) & mask) + total;
                 if (W >= 8)
-                    ottt_words<8, TB, true, T, A>(acc, E, tab, 2 * total, kv, le * W, 63, W, j0, 0, 64);
+                    ottt_words<8, TB, true, true, T, A>(acc, E, tab, 2 * total, kv, le * W, 63, W, j0, 0, 64);
                 else if (W == 4)
-                    ottt_words<4, TB, true, T, A>(acc, E, tab, 2 * total, kv, le * W, 63, W, j0, 0, 64);
+                    ottt_words<4, TB, true, true, T, A>(acc, E, tab, 2 * total, kv, le * W, 63, W, j0, 0, 64);
                 else if (W == 2)
-                    ottt_words<2, TB, true, T, A>(acc, E, tab, 2 * total, kv, le * W, 63, W, j0, 0, 64);
+                    ottt_words<2, TB, true, true, T, A>(acc, E, tab, 2 * total, kv, le * W, 63, W, j0, 0, 64);
                 else
-                    ottt_words<1, TB, true, T, A>(acc, E, tab, 2 * total, kv, le * W, 63, W, j0, 0, 64);
+                    ottt_words<1, TB, true, true, T, A>(acc, E, tab, 2 * total, kv, le * W, 63, W, j0, 0, 64);
 #pragma unroll
                 for (int t = 0; t < kOtttTables; t++)
                     if (t < TB) acc[t] = ottt_wave_sum(acc[t]);
@@ -323,13 +329,14 @@ __global__ __launch_bounds__(1024) void ottt_eval_chunked_kernel(T *__restrict__
     for (size_t base = (size_t)blockIdx.x * batch; base < N; base += (size_t)gridDim.x * batch)
     {
         A        acc[kOtttEW][kOtttTables];
-        uint32_t E[kOtttEW];
+        uint32_t E[kOtttEW], Ev[kOtttEW]; // Ev: per lane, E by wave sum at the end
         uint32_t xv = 0; // lane w < kOtttEW: x of the wave's element w
         if (lane < kOtttEW && base + first + lane < N) xv = (uint32_t)(uint64_t)x[base + first + lane];
 #pragma unroll
         for (int w = 0; w < kOtttEW; w++)
         {
-            E[w] = 0;
+            E[w]  = 0;
+            Ev[w] = 0;
 #pragma unroll
             for (int t = 0; t < kOtttTables; t++) acc[w][t] = 0;
         }
@@ -359,12 +366,16 @@ __global__ __launch_bounds__(1024) void ottt_eval_chunked_kernel(T *__restrict__
                 {
                     const uint32_t cnt = min(64u, nw - 1 - blk * 64);
                     const uint64_t kv  = lane < cnt ? ottt_load_word(kb, (k0 + blk * 64 + lane) & (W - 1)) : 0;
-                    ottt_words<8, TB, false, T, A>(acc[w], E[w], tab, stride, kv, 0, 63, cnt,
-                                                         j0 - 64u * 64u * blk, blk == 0 ? off : 0, 64);
+                    ottt_words<8, TB, false, false, T, A>(acc[w], E[w], tab, stride, kv, 0, 63, cnt,
+                                                                j0 - 64u * 64u * blk, blk == 0 ? off : 0, 64);
+                    // the run's set bits, counted in the lanes that hold its words (the
+                    // first word masked as above)
+                    Ev[w] += (uint32_t)__popcll(blk == 0 && lane == 0 ? kv & (~0ull << off) : kv);
                 }
                 const uint64_t kl = lane == 0 ? ottt_load_word(kb, (k0 + nw - 1) & (W - 1)) : 0;
-                ottt_words<1, TB, false, T, A>(acc[w], E[w], tab, stride, kl, 0, 63, 1,
-                                                     j0 - 64u * (nw - 1), 0, off);
+                ottt_words<1, TB, false, false, T, A>(acc[w], E[w], tab, stride, kl, 0, 63, 1,
+                                                            j0 - 64u * (nw - 1), 0, off);
+                Ev[w] += (uint32_t)__popcll(kl & ((1ull << off) - 1)); // off < 64
             }
         }
 #pragma unroll
@@ -373,6 +384,7 @@ __global__ __launch_bounds__(1024) void ottt_eval_chunked_kernel(T *__restrict__
 #pragma unroll
             for (int t = 0; t < kOtttTables; t++)
                 if (t < TB) acc[w][t] = ottt_wave_sum(acc[w][t]);
+            E[w] = ottt_wave_sum(Ev[w]);
             const size_t e = base + first + w;
             if (lane == 0 && e < N) ottt_write<T, A>(outE, outT, e, lutNum, t0, TB, party, writeE, acc[w], E[w]);
         }
